@@ -1,0 +1,246 @@
+/* libprima_hip: MI355X (gfx950) backend for Prima's restarted-PDHG LP solver.
+ *
+ * A plain C ABI: opaque handles, plain pointers and sizes, int status codes
+ * (0 = ok, negative = error; the message is kept per context and read with
+ * prima_last_error). Nothing is thrown across the ABI. Host pointers are
+ * borrowed only for the duration of a call.
+ *
+ * Two layers, matching the two seams of the reference (paths relative to
+ * modules/prima-core/src/main/scala/org/noaidi/prima/):
+ *
+ *  1. The per-operation kernel seam, `trait Kernels` (kernels/Kernels.scala:
+ *     42-125): allocate / upload / download / uploadMatrix / length / close
+ *     plus the eight numeric operations. A context of precision 64 behaves
+ *     like ScalaKernels (kernels/ScalaKernels.scala), one of precision 32 like
+ *     Float32Kernels (kernels/Float32Kernels.scala: values narrowed on upload,
+ *     infinities preserved, capabilities.supportsFloat64 = false).
+ *
+ *  2. The coarse on-device solve, which replaces the host loop of
+ *     Pdhg.solveWith (Pdhg.scala:48-55), MixedPrecision.solve
+ *     (MixedPrecision.scala:104-199) and the fan-out of PrimaZio.solveAll
+ *     (modules/prima-zio/.../PrimaZio.scala:61-66): the whole restarted-PDHG
+ *     iteration, KKT checkpoints, restarts and certificates run on the GPU.
+ *
+ * Threading: a context is not thread-safe and is used from one thread
+ * (Kernels.scala:39-40); prima_cancel is the one call that may come from
+ * another thread.
+ */
+#ifndef PRIMA_HIP_H
+#define PRIMA_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+#define PRIMA_OK 0
+#define PRIMA_EINVAL (-1)   /* IllegalArgumentException in the reference (require) */
+#define PRIMA_ENOMEM (-2)
+#define PRIMA_EDEVICE (-3)
+#define PRIMA_ECLOSED (-4)
+
+/* SolveStatus (SolveResult.scala:9-26), same ordinals. */
+#define PRIMA_OPTIMAL 0
+#define PRIMA_PRIMAL_INFEASIBLE 1
+#define PRIMA_DUAL_INFEASIBLE 2
+#define PRIMA_ITERATION_LIMIT 3
+#define PRIMA_TIME_LIMIT 4
+#define PRIMA_NUMERICAL_ERROR 5
+#define PRIMA_INTERRUPTED 6
+
+typedef struct prima_ctx prima_ctx;
+typedef struct prima_vec prima_vec;
+typedef struct prima_mat prima_mat;
+
+/* KernelCapabilities (kernels/Kernels.scala:13-22). */
+typedef struct {
+  char name[32];
+  char device[64];
+  int32_t supports_float64;
+} prima_capabilities;
+
+/* ---- context ------------------------------------------------------------ */
+
+/* Open a context on a HIP device with iteration precision 64 or 32.
+ * Replaces constructing a Kernels backend (ScalaKernels() /
+ * Float32Kernels(), kernels/ScalaKernels.scala:125, Float32Kernels.scala:157). */
+int prima_open(int32_t device, int32_t precision, prima_ctx** out);
+/* Kernels.close (kernels/Kernels.scala:65-66): releases device resources;
+ * idempotent. The handle stays valid (calls then return PRIMA_ECLOSED)
+ * until prima_destroy. */
+int prima_close(prima_ctx* ctx);
+/* Frees the handle itself (implies close). */
+void prima_destroy(prima_ctx* ctx);
+/* Kernels.capabilities (kernels/Kernels.scala:47). */
+int prima_capabilities_of(const prima_ctx* ctx, prima_capabilities* out);
+/* Message for the last failing call on this context ("" if none). */
+const char* prima_last_error(const prima_ctx* ctx);
+
+/* ---- memory (kernels/Kernels.scala:51-63) -------------------------------- */
+
+/* Kernels.allocate: a zero-filled vector of length n. */
+int prima_vec_alloc(prima_ctx* ctx, int64_t n, prima_vec** out);
+/* Kernels.upload: copies (never aliases) host data; narrows to float in a
+ * precision-32 context with infinities preserved. */
+int prima_vec_upload(prima_ctx* ctx, const double* data, int64_t n, prima_vec** out);
+/* Kernels.download: into a caller-owned array of length prima_vec_length. */
+int prima_vec_download(prima_ctx* ctx, const prima_vec* v, double* into, int64_t n);
+/* Kernels.length. */
+int64_t prima_vec_length(const prima_vec* v);
+int prima_vec_free(prima_ctx* ctx, prima_vec* v);
+/* Kernels.uploadMatrix from canonical CSR (SparseMatrix.scala:15-21): rows
+ * of row_ptr ascending, column indices in range, values finite. */
+int prima_mat_upload_csr(prima_ctx* ctx, int32_t rows, int32_t cols, int64_t nnz,
+                         const int32_t* row_ptr, const int32_t* col, const double* vals,
+                         prima_mat** out);
+int prima_mat_free(prima_ctx* ctx, prima_mat* m);
+
+/* ---- the eight operations (kernels/Kernels.scala:73-123) ---------------- */
+
+/* Kernels.spmv: out := a * x. */
+int prima_spmv(prima_ctx* ctx, const prima_mat* a, const prima_vec* x, prima_vec* out);
+/* Kernels.axpby: out := alpha x + beta y; out may alias x or y. */
+int prima_axpby(prima_ctx* ctx, double alpha, const prima_vec* x, double beta,
+                const prima_vec* y, prima_vec* out);
+/* Kernels.scale: out := alpha x. */
+int prima_scale(prima_ctx* ctx, double alpha, const prima_vec* x, prima_vec* out);
+/* Kernels.copy. */
+int prima_copy(prima_ctx* ctx, const prima_vec* src, prima_vec* dst);
+/* Kernels.dot; synchronises the context's stream. */
+int prima_dot(prima_ctx* ctx, const prima_vec* x, const prima_vec* y, double* out);
+/* Kernels.squaredNorm; synchronises the context's stream. */
+int prima_squared_norm(prima_ctx* ctx, const prima_vec* x, double* out);
+/* Kernels.primalStep: out := clamp(x - tau (cost - ktY), lower, upper). */
+int prima_primal_step(prima_ctx* ctx, const prima_vec* x, const prima_vec* kty,
+                      const prima_vec* cost, const prima_vec* lower, const prima_vec* upper,
+                      double tau, prima_vec* out);
+/* Kernels.dualStep: out := proj(y + sigma (rhs - kxBar)), first n_eq free. */
+int prima_dual_step(prima_ctx* ctx, const prima_vec* y, const prima_vec* kx_bar,
+                    const prima_vec* rhs, double sigma, int32_t n_eq, prima_vec* out);
+
+/* ---- coarse on-device solve --------------------------------------------- */
+
+/* LpProblem in standard form (LpProblem.scala:22-104): min c'x + offset,
+ * K x = q on the first n_eq rows, K x >= q on the rest, l <= x <= u.
+ * K is canonical CSR, m x n. */
+typedef struct {
+  int32_t n, m, n_eq;
+  int32_t pad;
+  int64_t nnz;
+  const int32_t* row_ptr;
+  const int32_t* col;
+  const double* val;
+  const double* c;
+  const double* q;
+  const double* l;
+  const double* u;
+  double offset;
+} prima_lp;
+
+/* PdhgParams + ScalingParams (PdhgParams.scala:9-59, Scaling.scala:11-24),
+ * plus launch tuning that has no counterpart in the reference. */
+typedef struct {
+  double eps_abs;
+  double eps_rel;
+  int32_t max_iterations;
+  int32_t evaluation_frequency;
+  int64_t time_limit_ms; /* < 0: none */
+  int32_t ruiz_iterations;
+  int32_t apply_pock_chambolle;
+  double pock_chambolle_alpha;
+  double infeasibility_tolerance;
+  double restart_sufficient;
+  double restart_necessary;
+  double restart_artificial;
+  double primal_weight_smoothing;
+  int32_t workgroups_per_lp; /* 0: chosen by the tuner */
+  int32_t chunk_iterations;  /* iterations per launch; 0: adaptive */
+} prima_params;
+
+/* Pdhg.WarmStart (Pdhg.scala:72-77) in the caller's coordinates; a step
+ * size or primal weight that is not finite and positive means "not given". */
+typedef struct {
+  const double* primal; /* n */
+  const double* dual;   /* m */
+  double step_size;
+  double primal_weight;
+} prima_warm;
+
+/* LpSolution scalars (SolveResult.scala:44-65) with its KktError. */
+typedef struct {
+  int32_t status;
+  int32_t iterations;
+  int32_t restarts;
+  int32_t used_average;
+  double objective;
+  double dual_objective;
+  double primal_residual;
+  double dual_residual;
+  double solve_time_ms;  /* iteration time, after scaling and upload (Pdhg.scala:204) */
+  double total_time_ms;  /* including scaling, transpose and upload */
+  double final_step_size;
+  double final_primal_weight;
+  int64_t trials;        /* step-size trials, accepted and rejected */
+  int32_t evaluations;   /* checkpoints */
+  int32_t workgroups;    /* workgroups that owned this LP */
+} prima_result;
+
+/* MixedPrecision.Params (MixedPrecision.scala:37-76). */
+typedef struct {
+  double reduced_tolerance;
+  int32_t reduced_max_iterations;
+  int32_t always_refine;
+  double reduced_time_share;
+} prima_mixed_params;
+
+/* MixedPrecision.Result scalars (MixedPrecision.scala:82-91). */
+typedef struct {
+  int32_t reduced_iterations;
+  int32_t reduced_restarts;
+  int32_t refinement_iterations;
+  int32_t refined;
+  double reduced_ms;
+  double refinement_ms;
+} prima_mixed_info;
+
+void prima_default_params(prima_params* p);
+void prima_default_mixed_params(prima_mixed_params* p);
+
+/* Pdhg.solveWith(problem, params, kernels, cancelled, warmStart)
+ * (Pdhg.scala:48-55) on this context's precision, entirely on the device.
+ * warm may be NULL; x_out (n), y_out (m), rc_out (n) may be NULL. */
+int prima_solve(prima_ctx* ctx, const prima_lp* lp, const prima_params* params,
+                const prima_warm* warm, prima_result* res, double* x_out, double* y_out,
+                double* rc_out);
+
+/* MixedPrecision.solve (MixedPrecision.scala:104-199). A precision-64
+ * context skips the reduced pass (MixedPrecision.scala:111-113); a
+ * precision-32 context runs the float32 pass on the device, re-checks a
+ * certificate in double precision and finishes with a float64 device pass
+ * warm-started from (x, y, step size, primal weight). */
+int prima_mixed_solve(prima_ctx* ctx, const prima_lp* lp, const prima_params* params,
+                      const prima_mixed_params* mixed, prima_result* res, prima_mixed_info* info,
+                      double* x_out, double* y_out, double* rc_out);
+
+/* Independent LPs solved together (PrimaZio.solveAll, PrimaZio.scala:61-66):
+ * each LP gets its own workgroups, step size, restarts and termination inside
+ * shared persistent launches. Results keep input order. warms may be NULL or
+ * hold NULL entries; x_outs/y_outs/rc_outs may be NULL or hold NULL entries.
+ * mixed may be NULL (plain solve in the context's precision); otherwise each
+ * LP follows prima_mixed_solve's two-pass contract. */
+int prima_solve_batch(prima_ctx* ctx, int32_t count, const prima_lp* lps,
+                      const prima_params* params, const prima_mixed_params* mixed,
+                      const prima_warm* warms, prima_result* results, prima_mixed_info* infos,
+                      double** x_outs, double** y_outs, double** rc_outs);
+
+/* Cooperative cancellation (the `cancelled` callback of Pdhg.scala:43-47):
+ * a running solve on ctx stops at its next launch boundary with status
+ * PRIMA_INTERRUPTED. Safe to call from another thread. */
+int prima_cancel(prima_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1294 @@
+// C-ABI of libprima_hip (include/prima_hip.h) and the host side of the
+// device solve: validation, equilibration, transpose, SpMV row blocking,
+// workgroup partitioning, device memory layout and the chunked persistent
+// launches. Reference citations are relative to
+// modules/prima-core/src/main/scala/org/noaidi/prima/.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/prima_hip.h"
+#include "pdhg_shared.h"
+
+namespace prima {
+// pdhg_kernel.hip
+int occupancy_blocks_per_cu(int precision);
+hipError_t launch_pdhg(int precision, const LpView* views, const int32_t* wg_lp, int grid,
+                       const KParams& P, hipStream_t stream);
+int threads_per_block();
+int tile_nnz();
+// kernels_ops.hip
+hipError_t op_spmv(int prec, int rows, int64_t nnz, const int32_t* ptr, const int32_t* col,
+                   const void* val, const void* x, void* out, hipStream_t s);
+hipError_t op_axpby(int prec, int64_t n, double a, const void* x, double b, const void* y,
+                    void* out, hipStream_t s);
+hipError_t op_scale(int prec, int64_t n, double a, const void* x, void* out, hipStream_t s);
+hipError_t op_primal(int prec, int64_t n, const void* x, const void* kty, const void* c,
+                     const void* l, const void* u, double tau, void* out, hipStream_t s);
+hipError_t op_dual(int prec, int64_t m, const void* y, const void* kxb, const void* q,
+                   double sigma, int64_t n_eq, void* out, hipStream_t s);
+hipError_t op_dot(int prec, int64_t n, const void* x, const void* y, double* part,
+                  hipStream_t s);
+}  // namespace prima
+
+using namespace prima;
+
+struct prima_ctx {
+  int device = 0;
+  int precision = 64;
+  bool closed = false;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int cu_count = 0;
+  int occ[2] = {0, 0};  // blocks per CU for float / double kernels
+  double* scratch = nullptr;  // reduction partials for dot / squaredNorm
+  std::atomic<int> cancel{0};
+  std::string device_name;
+  bool coop_ok = false;
+};
+
+struct prima_vec {
+  void* d = nullptr;
+  int64_t n = 0;
+};
+
+struct prima_mat {
+  int32_t rows = 0, cols = 0;
+  int64_t nnz = 0;
+  int32_t* ptr = nullptr;
+  int32_t* col = nullptr;
+  void* val = nullptr;
+};
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+double ms_since(clk::time_point t0) {
+  return std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+}
+
+int fail(prima_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+int dev_fail(prima_ctx* ctx, hipError_t e, const char* what) {
+  return fail(ctx, PRIMA_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(ctx, expr)                                \
+  do {                                                    \
+    hipError_t _e = (expr);                               \
+    if (_e != hipSuccess) return dev_fail(ctx, _e, #expr); \
+  } while (0)
+
+size_t esize(int prec) { return prec == 32 ? sizeof(float) : sizeof(double); }
+
+// Narrow to the iteration precision; infinities survive
+// (Float32Kernels.scala:43-52).
+void to_prec(int prec, const double* src, int64_t n, void* dst) {
+  if (prec == 32) {
+    float* f = (float*)dst;
+    for (int64_t i = 0; i < n; ++i) f[i] = (float)src[i];
+  } else {
+    std::memcpy(dst, src, (size_t)n * sizeof(double));
+  }
+}
+
+int check_open(prima_ctx* ctx) {
+  if (!ctx) return PRIMA_EINVAL;
+  if (ctx->closed) return fail(ctx, PRIMA_ECLOSED, "context is closed");
+  ctx->err.clear();
+  if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, PRIMA_EDEVICE, "hipSetDevice");
+  return PRIMA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Host preprocessing of one LP.
+
+struct HostLp {
+  int32_t n = 0, m = 0, n_eq = 0;
+  int64_t nnz = 0;
+  const prima_lp* src = nullptr;
+  std::vector<double> aval;  // scaled values, A order
+  std::vector<int32_t> tptr, tcol;
+  std::vector<double> tval;  // scaled transpose
+  std::vector<double> rs, cs, sc, sq, sl, su;
+  double spec_scaled = 0, spec_orig = 0, rhs_norm = 0, obj_norm = 0;
+  std::vector<int32_t> a_rb, t_rb;
+  int G = 1;
+};
+
+// LpProblem.apply requirements (LpProblem.scala:62-104) and canonical CSR
+// (SparseMatrix.scala:15-21, 204-260).
+std::string validate_lp(const prima_lp* lp) {
+  if (lp->n < 0 || lp->m < 0) return "negative dimensions";
+  if (lp->n_eq < 0 || lp->n_eq > lp->m) return "numEqualities out of range";
+  if (lp->m > 0 && !lp->row_ptr) return "row_ptr is null";
+  if (lp->nnz > 0 && (!lp->col || !lp->val)) return "col or val is null";
+  if (lp->n > 0 && (!lp->c || !lp->l || !lp->u)) return "c, l or u is null";
+  if (lp->m > 0 && !lp->q) return "q is null";
+  if (lp->nnz > INT32_MAX) return "nnz exceeds int32 indices";
+  if (lp->m > 0) {
+    if (lp->row_ptr[0] != 0 || lp->row_ptr[lp->m] != lp->nnz) return "row_ptr does not span nnz";
+    for (int32_t r = 0; r < lp->m; ++r) {
+      if (lp->row_ptr[r + 1] < lp->row_ptr[r]) return "row_ptr is not monotone";
+      for (int32_t p = lp->row_ptr[r]; p < lp->row_ptr[r + 1]; ++p) {
+        if (lp->col[p] < 0 || lp->col[p] >= lp->n) return "column index out of range";
+        if (p > lp->row_ptr[r] && lp->col[p] <= lp->col[p - 1])
+          return "column indices not strictly ascending within a row";
+        if (!std::isfinite(lp->val[p])) return "matrix entry is not finite";
+      }
+    }
+  } else if (lp->nnz != 0) {
+    return "nnz without rows";
+  }
+  for (int32_t i = 0; i < lp->n; ++i) {
+    if (!(lp->l[i] <= lp->u[i])) return "empty bound interval for variable " + std::to_string(i);
+    if (!std::isfinite(lp->c[i])) return "objective coefficient is not finite";
+  }
+  for (int32_t r = 0; r < lp->m; ++r)
+    if (!std::isfinite(lp->q[r])) return "right-hand side is not finite";
+  if (!std::isfinite(lp->offset)) return "objective offset is not finite";
+  return "";
+}
+
+// PdhgParams / ScalingParams requirements (PdhgParams.scala:61-86,
+// Scaling.scala:16-20).
+std::string validate_params(const prima_params* p) {
+  if (!(p->eps_abs >= 0.0)) return "epsAbs must be non-negative";
+  if (!(p->eps_rel >= 0.0)) return "epsRel must be non-negative";
+  if (p->max_iterations <= 0) return "maxIterations must be positive";
+  if (p->evaluation_frequency <= 0) return "evaluationFrequency must be positive";
+  if (!(p->primal_weight_smoothing >= 0.0 && p->primal_weight_smoothing <= 1.0))
+    return "primalWeightSmoothing must lie in [0, 1]";
+  if (!(p->restart_sufficient > 0.0 && p->restart_sufficient <= p->restart_necessary))
+    return "restartSufficientReduction must be positive and no greater than "
+           "restartNecessaryReduction";
+  if (!(p->restart_artificial > 0.0 && p->restart_artificial <= 1.0))
+    return "restartArtificialFraction must lie in (0, 1]";
+  if (!(p->infeasibility_tolerance >= 0.0)) return "infeasibilityTolerance must be non-negative";
+  if (p->ruiz_iterations < 0) return "ruizIterations must be non-negative";
+  if (!(p->pock_chambolle_alpha >= 0.0 && p->pock_chambolle_alpha <= 2.0))
+    return "pockChambolleAlpha must lie in [0, 2]";
+  if (p->workgroups_per_lp < 0 || p->chunk_iterations < 0) return "negative launch tuning";
+  return "";
+}
+
+void transpose_csr(int32_t rows, int32_t cols, const int32_t* ptr, const int32_t* col,
+                   const double* val, std::vector<int32_t>& tptr, std::vector<int32_t>& tcol,
+                   std::vector<double>& tval) {
+  const int64_t nnz = rows > 0 ? ptr[rows] : 0;
+  tptr.assign((size_t)cols + 1, 0);
+  tcol.resize((size_t)nnz);
+  tval.resize((size_t)nnz);
+  for (int64_t k = 0; k < nnz; ++k) tptr[col[k] + 1]++;
+  for (int32_t c = 0; c < cols; ++c) tptr[c + 1] += tptr[c];
+  std::vector<int32_t> cur(tptr.begin(), tptr.end() - 1);
+  for (int32_t r = 0; r < rows; ++r)
+    for (int32_t p = ptr[r]; p < ptr[r + 1]; ++p) {
+      const int32_t d = cur[col[p]]++;
+      tcol[d] = r;
+      tval[d] = val[p];
+    }
+}
+
+double norm_inf(int32_t rows, const int32_t* ptr, const double* val) {
+  double mx = 0.0;
+  for (int32_t r = 0; r < rows; ++r) {
+    double s = 0.0;
+    for (int32_t p = ptr[r]; p < ptr[r + 1]; ++p) s += std::fabs(val[p]);
+    mx = std::max(mx, s);
+  }
+  return mx;
+}
+
+double norm2(const double* a, int64_t n) {
+  double s = 0.0;
+  for (int64_t i = 0; i < n; ++i) s += a[i] * a[i];
+  return std::sqrt(s);
+}
+
+// Scaling.apply (Scaling.scala:77-155): Ruiz passes then Pock-Chambolle.
+void equilibrate(const prima_lp* lp, const prima_params* p, HostLp& h) {
+  const int32_t m = lp->m, n = lp->n;
+  const int64_t nnz = lp->nnz;
+  h.aval.assign(lp->val, lp->val + nnz);
+  h.rs.assign((size_t)m, 1.0);
+  h.cs.assign((size_t)n, 1.0);
+  std::vector<double> rf((size_t)m), cf((size_t)n);
+  auto apply = [&]() {
+    for (int32_t r = 0; r < m; ++r)
+      for (int32_t q = lp->row_ptr[r]; q < lp->row_ptr[r + 1]; ++q)
+        h.aval[q] = h.aval[q] * rf[r] * cf[lp->col[q]];
+    for (int32_t r = 0; r < m; ++r) h.rs[r] *= rf[r];
+    for (int32_t c = 0; c < n; ++c) h.cs[c] *= cf[c];
+  };
+  for (int it = 0; it < p->ruiz_iterations; ++it) {
+    std::fill(cf.begin(), cf.end(), 0.0);
+    for (int32_t r = 0; r < m; ++r) {
+      double mx = 0.0;
+      for (int32_t q = lp->row_ptr[r]; q < lp->row_ptr[r + 1]; ++q) {
+        const double a = std::fabs(h.aval[q]);
+        mx = std::max(mx, a);
+        double& cm = cf[lp->col[q]];
+        if (a > cm) cm = a;
+      }
+      rf[r] = mx > 0.0 ? 1.0 / std::sqrt(mx) : 1.0;
+    }
+    for (int32_t c = 0; c < n; ++c) cf[c] = cf[c] > 0.0 ? 1.0 / std::sqrt(cf[c]) : 1.0;
+    apply();
+  }
+  if (p->apply_pock_chambolle) {
+    const double alpha = p->pock_chambolle_alpha;
+    std::fill(cf.begin(), cf.end(), 0.0);
+    for (int32_t r = 0; r < m; ++r) {
+      double s = 0.0;
+      for (int32_t q = lp->row_ptr[r]; q < lp->row_ptr[r + 1]; ++q) {
+        const double a = std::fabs(h.aval[q]);
+        s += alpha == 1.0 ? a : std::pow(a, 2.0 - alpha);
+        cf[lp->col[q]] += alpha == 1.0 ? a : std::pow(a, alpha);
+      }
+      rf[r] = s > 0.0 ? 1.0 / std::sqrt(s) : 1.0;
+    }
+    for (int32_t c = 0; c < n; ++c) cf[c] = cf[c] > 0.0 ? 1.0 / std::sqrt(cf[c]) : 1.0;
+    apply();
+  }
+  h.sc.resize(n);
+  h.sl.resize(n);
+  h.su.resize(n);
+  h.sq.resize(m);
+  for (int32_t i = 0; i < n; ++i) {
+    const double d = h.cs[i];
+    h.sc[i] = lp->c[i] * d;
+    h.sl[i] = lp->l[i] / d;
+    h.su[i] = lp->u[i] / d;
+  }
+  for (int32_t r = 0; r < m; ++r) h.sq[r] = lp->q[r] * h.rs[r];
+}
+
+// Row blocks for the streamed SpMV: consecutive rows while their nnz fit the
+// LDS tile and there are at most threads_per_block of them; a row longer
+// than the tile is a block of its own.
+std::vector<int32_t> row_blocks(int32_t rows, const int32_t* ptr) {
+  const int tile = tile_nnz(), nt = threads_per_block();
+  std::vector<int32_t> rb;
+  rb.push_back(0);
+  int32_t r = 0;
+  while (r < rows) {
+    const int32_t start = r;
+    if (ptr[r + 1] - ptr[r] > tile) {
+      ++r;
+    } else {
+      int64_t acc = 0;
+      while (r < rows && r - start < nt && acc + (ptr[r + 1] - ptr[r]) <= tile) {
+        acc += ptr[r + 1] - ptr[r];
+        ++r;
+      }
+    }
+    rb.push_back(r);
+  }
+  return rb;
+}
+
+// Split row blocks into G contiguous ranges of near-equal cost (nnz plus a
+// per-row charge for the fused elementwise work).
+std::vector<int32_t> partition(const std::vector<int32_t>& rb, const int32_t* ptr, int G) {
+  const size_t nb = rb.size() - 1;
+  std::vector<double> pre(nb + 1, 0.0);
+  for (size_t b = 0; b < nb; ++b) {
+    const double nz = (double)(ptr[rb[b + 1]] - ptr[rb[b]]);
+    pre[b + 1] = pre[b] + nz + 4.0 * (rb[b + 1] - rb[b]) + 16.0;
+  }
+  std::vector<int32_t> w((size_t)G + 1);
+  w[0] = 0;
+  for (int g = 1; g < G; ++g) {
+    const double target = pre[nb] * g / G;
+    w[g] = (int32_t)(std::lower_bound(pre.begin(), pre.end(), target) - pre.begin());
+    if (w[g] < w[g - 1]) w[g] = w[g - 1];
+    if (w[g] > (int32_t)nb) w[g] = (int32_t)nb;
+  }
+  w[G] = (int32_t)nb;
+  return w;
+}
+
+void prepare(const prima_lp* lp, const prima_params* p, HostLp& h) {
+  h.src = lp;
+  h.n = lp->n;
+  h.m = lp->m;
+  h.n_eq = lp->n_eq;
+  h.nnz = lp->nnz;
+  static const int32_t zero_ptr[1] = {0};
+  const int32_t* ptr = lp->m > 0 ? lp->row_ptr : zero_ptr;
+  equilibrate(lp, p, h);
+  transpose_csr(lp->m, lp->n, ptr, lp->col, h.aval.data(), h.tptr, h.tcol, h.tval);
+  // spectralNormBound = sqrt(||K||_1 ||K||_inf) (SparseMatrix.scala:181)
+  h.spec_scaled = std::sqrt(norm_inf(lp->n, h.tptr.data(), h.tval.data()) *
+                            norm_inf(lp->m, ptr, h.aval.data()));
+  {
+    std::vector<int32_t> optr, ocol;
+    std::vector<double> oval;
+    transpose_csr(lp->m, lp->n, ptr, lp->col, lp->val, optr, ocol, oval);
+    h.spec_orig = std::sqrt(norm_inf(lp->n, optr.data(), oval.data()) *
+                            norm_inf(lp->m, ptr, lp->val));
+  }
+  h.rhs_norm = norm2(lp->q, lp->m);
+  h.obj_norm = norm2(lp->c, lp->n);
+  h.a_rb = row_blocks(lp->m, ptr);
+  h.t_rb = row_blocks(lp->n, h.tptr.data());
+}
+
+// Certificates.classify (Certificates.scala:207-219) in double precision on
+// the original data, used to re-check a certificate found by a float32 pass
+// (MixedPrecision.scala:143-154). Returns -1 when neither holds.
+int host_classify(const prima_lp* lp, const double* x, const double* y, double tol) {
+  const int32_t n = lp->n, m = lp->m;
+  static const int32_t zero_ptr[1] = {0};
+  const int32_t* ptr = m > 0 ? lp->row_ptr : zero_ptr;
+  std::vector<int32_t> tptr, tcol;
+  std::vector<double> tval;
+  transpose_csr(m, n, ptr, lp->col, lp->val, tptr, tcol, tval);
+  const double spec =
+      std::sqrt(norm_inf(n, tptr.data(), tval.data()) * norm_inf(m, ptr, lp->val));
+  std::vector<double> aty(n, 0.0), ax(m, 0.0);
+  for (int32_t i = 0; i < n; ++i)
+    for (int32_t q = tptr[i]; q < tptr[i + 1]; ++q) aty[i] += tval[q] * y[tcol[q]];
+  for (int32_t r = 0; r < m; ++r)
+    for (int32_t q = ptr[r]; q < ptr[r + 1]; ++q) ax[r] += lp->val[q] * x[lp->col[q]];
+  auto terms = [](double rc, double lo, double hi, double& sq, double& lin) {
+    if (rc > 0.0) {
+      if (std::isinf(lo) && lo < 0) sq += rc * rc;
+      else lin += lo * rc;
+    } else if (rc < 0.0) {
+      if (std::isinf(hi) && hi > 0) sq += rc * rc;
+      else lin += hi * rc;
+    }
+  };
+  {  // primal infeasibility from y
+    double nsq = 0.0;
+    for (int32_t r = 0; r < m; ++r) nsq += y[r] * y[r];
+    if (nsq > 0.0 && std::isfinite(nsq)) {
+      const double norm = std::sqrt(nsq);
+      double value = 0.0, rsq = 0.0, csq = 0.0;
+      for (int32_t r = 0; r < m; ++r) value += lp->q[r] * y[r];
+      for (int32_t i = 0; i < n; ++i) terms(-aty[i], lp->l[i], lp->u[i], rsq, value);
+      for (int32_t r = lp->n_eq; r < m; ++r)
+        if (y[r] < 0.0) csq += y[r] * y[r];
+      if (value > 0.0 && std::isfinite(value)) {
+        const double rt = spec > 0.0 ? std::sqrt(rsq) / (spec * norm) : 0.0;
+        const double h = std::hypot(rt, std::sqrt(csq) / norm);
+        if (std::isfinite(h) && h <= tol) return PRIMA_PRIMAL_INFEASIBLE;
+      }
+    }
+  }
+  {  // unboundedness from x
+    double nsq = 0.0, value = 0.0;
+    for (int32_t i = 0; i < n; ++i) {
+      nsq += x[i] * x[i];
+      value += lp->c[i] * x[i];
+    }
+    if (nsq > 0.0 && std::isfinite(nsq) && value < 0.0 && std::isfinite(value)) {
+      const double norm = std::sqrt(nsq);
+      double rsq = 0.0, bsq = 0.0;
+      for (int32_t r = 0; r < m; ++r)
+        if (r < lp->n_eq || ax[r] < 0.0) rsq += ax[r] * ax[r];
+      for (int32_t i = 0; i < n; ++i) {
+        if (x[i] > 0.0 && !(std::isinf(lp->u[i]) && lp->u[i] > 0)) bsq += x[i] * x[i];
+        else if (x[i] < 0.0 && !(std::isinf(lp->l[i]) && lp->l[i] < 0)) bsq += x[i] * x[i];
+      }
+      const double rt = spec > 0.0 ? std::sqrt(rsq) / (spec * norm) : 0.0;
+      const double h = std::hypot(rt, std::sqrt(bsq) / norm);
+      if (std::isfinite(h) && h <= tol) return PRIMA_DUAL_INFEASIBLE;
+    }
+  }
+  return -1;
+}
+
+// ---------------------------------------------------------------------------
+// Device memory for a wave of LPs: one arena, carved into 256-byte aligned
+// arrays.
+
+struct Carver {
+  size_t off = 0;
+  size_t take(size_t bytes) {
+    const size_t o = off;
+    off += (bytes + 255) & ~(size_t)255;
+    return o;
+  }
+};
+
+struct LpLayout {
+  size_t a_ptr, a_col, t_ptr, t_col, a_rb, t_rb, a_wrb, t_wrb;
+  size_t a_val, t_val, c, q, l, u;
+  size_t x, y, kx, kty, xn, yn, xbar, kxbar, sx, sy, skx, skty, xr, yr;
+  size_t oc, oq, ol, ou, rs, cs, x_out, y_out, rc_out, part, bar, st;
+};
+
+struct Wave {
+  std::vector<int> idx;  // indices into the batch
+  std::vector<LpLayout> lay;
+  std::vector<LpView> views;
+  std::vector<int32_t> wg_lp;
+  char* arena = nullptr;
+  LpView* d_views = nullptr;
+  int32_t* d_wg_lp = nullptr;
+  int grid = 0;
+};
+
+LpLayout carve(Carver& cv, const HostLp& h, int prec) {
+  const size_t s = esize(prec), n = (size_t)h.n, m = (size_t)h.m, z = (size_t)h.nnz;
+  const size_t G = (size_t)h.G;
+  LpLayout L;
+  L.a_ptr = cv.take((m + 1) * 4);
+  L.a_col = cv.take(z * 4);
+  L.t_ptr = cv.take((n + 1) * 4);
+  L.t_col = cv.take(z * 4);
+  L.a_rb = cv.take(h.a_rb.size() * 4);
+  L.t_rb = cv.take(h.t_rb.size() * 4);
+  L.a_wrb = cv.take((G + 1) * 4);
+  L.t_wrb = cv.take((G + 1) * 4);
+  L.a_val = cv.take(z * s);
+  L.t_val = cv.take(z * s);
+  L.c = cv.take(n * s);
+  L.l = cv.take(n * s);
+  L.u = cv.take(n * s);
+  L.q = cv.take(m * s);
+  L.x = cv.take(n * s);
+  L.kty = cv.take(n * s);
+  L.xn = cv.take(n * s);
+  L.xbar = cv.take(n * s);
+  L.sx = cv.take(n * s);
+  L.skty = cv.take(n * s);
+  L.xr = cv.take(n * s);
+  L.y = cv.take(m * s);
+  L.kx = cv.take(m * s);
+  L.yn = cv.take(m * s);
+  L.kxbar = cv.take(m * s);
+  L.sy = cv.take(m * s);
+  L.skx = cv.take(m * s);
+  L.yr = cv.take(m * s);
+  L.oc = cv.take(n * 8);
+  L.ol = cv.take(n * 8);
+  L.ou = cv.take(n * 8);
+  L.cs = cv.take(n * 8);
+  L.x_out = cv.take(n * 8);
+  L.rc_out = cv.take(n * 8);
+  L.oq = cv.take(m * 8);
+  L.rs = cv.take(m * 8);
+  L.y_out = cv.take(m * 8);
+  L.part = cv.take((2 * G * kIterParts + G * kEvalParts) * 8);
+  L.bar = cv.take(16);
+  L.st = cv.take(sizeof(SolveState));
+  return L;
+}
+
+struct SolveJob {
+  const prima_lp* lp = nullptr;
+  HostLp h;
+  const prima_warm* warm = nullptr;
+  prima_result* res = nullptr;
+  double *x_out = nullptr, *y_out = nullptr, *rc_out = nullptr;
+  double prep_ms = 0;
+};
+
+int choose_groups(prima_ctx* ctx, const prima_params* p, int64_t nnz, int capacity) {
+  if (p->workgroups_per_lp > 0) return std::min(p->workgroups_per_lp, capacity);
+  // enough rows per workgroup to hide memory latency, few enough workgroups
+  // that barrier latency does not dominate small LPs
+  int g = (int)((nnz + 24575) / 24576);
+  return std::max(1, std::min(g, capacity));
+}
+
+// Upload one wave, run it to completion in chunks, download the results.
+int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::vector<int>& idx,
+             const prima_params* p) {
+  Wave W;
+  W.idx = idx;
+  Carver cv;
+  for (int j : idx) W.lay.push_back(carve(cv, jobs[j].h, prec));
+  const size_t arena_bytes = cv.off;
+  int grid = 0;
+  for (int j : idx) grid += jobs[j].h.G;
+  W.grid = grid;
+
+  auto t_up = clk::now();
+  void* arena = nullptr;
+  hipError_t e = hipMalloc(&arena, arena_bytes);
+  if (e != hipSuccess) return fail(ctx, PRIMA_ENOMEM, "device arena of " +
+                                             std::to_string(arena_bytes) + " bytes");
+  W.arena = (char*)arena;
+  struct Guard {
+    Wave& w;
+    ~Guard() {
+      if (w.arena) (void)hipFree(w.arena);
+      if (w.d_views) (void)hipFree(w.d_views);
+      if (w.d_wg_lp) (void)hipFree(w.d_wg_lp);
+    }
+  } guard{W};
+  HIP_TRY(ctx, hipMemsetAsync(arena, 0, arena_bytes, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+
+  // Uploads are synchronous copies: host buffers may be released as soon as
+  // each call returns.
+  const size_t s = esize(prec);
+  auto put = [&](size_t off, const void* src, size_t bytes) -> hipError_t {
+    if (bytes == 0) return hipSuccess;
+    return hipMemcpy(W.arena + off, src, bytes, hipMemcpyHostToDevice);
+  };
+  std::vector<char> conv;
+  auto put_t = [&](size_t off, const double* src, size_t count) -> hipError_t {
+    if (count == 0) return hipSuccess;
+    if (prec == 64) return put(off, src, count * 8);
+    conv.resize(count * s);
+    to_prec(prec, src, (int64_t)count, conv.data());
+    return put(off, conv.data(), count * s);
+  };
+
+  int wg_base = 0;
+  W.wg_lp.resize(grid);
+  for (size_t k = 0; k < idx.size(); ++k) {
+    SolveJob& J = jobs[idx[k]];
+    HostLp& h = J.h;
+    const prima_lp* lp = J.lp;
+    const LpLayout& L = W.lay[k];
+    static const int32_t zero_ptr[1] = {0};
+    const int32_t* aptr = lp->m > 0 ? lp->row_ptr : zero_ptr;
+    std::vector<int32_t> awrb = partition(h.a_rb, aptr, h.G);
+    std::vector<int32_t> twrb = partition(h.t_rb, h.tptr.data(), h.G);
+    HIP_TRY(ctx, put(L.a_wrb, awrb.data(), awrb.size() * 4));
+    HIP_TRY(ctx, put(L.t_wrb, twrb.data(), twrb.size() * 4));
+    HIP_TRY(ctx, put(L.a_ptr, aptr, ((size_t)h.m + 1) * 4));
+    HIP_TRY(ctx, put(L.a_col, lp->col, (size_t)h.nnz * 4));
+    HIP_TRY(ctx, put(L.t_ptr, h.tptr.data(), ((size_t)h.n + 1) * 4));
+    HIP_TRY(ctx, put(L.t_col, h.tcol.data(), (size_t)h.nnz * 4));
+    HIP_TRY(ctx, put(L.a_rb, h.a_rb.data(), h.a_rb.size() * 4));
+    HIP_TRY(ctx, put(L.t_rb, h.t_rb.data(), h.t_rb.size() * 4));
+    HIP_TRY(ctx, put_t(L.a_val, h.aval.data(), (size_t)h.nnz));
+    HIP_TRY(ctx, put_t(L.t_val, h.tval.data(), (size_t)h.nnz));
+    HIP_TRY(ctx, put_t(L.c, h.sc.data(), h.n));
+    HIP_TRY(ctx, put_t(L.l, h.sl.data(), h.n));
+    HIP_TRY(ctx, put_t(L.u, h.su.data(), h.n));
+    HIP_TRY(ctx, put_t(L.q, h.sq.data(), h.m));
+    HIP_TRY(ctx, put(L.oc, lp->c, (size_t)h.n * 8));
+    HIP_TRY(ctx, put(L.ol, lp->l, (size_t)h.n * 8));
+    HIP_TRY(ctx, put(L.ou, lp->u, (size_t)h.n * 8));
+    HIP_TRY(ctx, put(L.cs, h.cs.data(), (size_t)h.n * 8));
+    HIP_TRY(ctx, put(L.oq, lp->q, (size_t)h.m * 8));
+    HIP_TRY(ctx, put(L.rs, h.rs.data(), (size_t)h.m * 8));
+
+    // Starting point (Pdhg.scala:265-303): x0 = proj(warm / colScale or 0),
+    // y0 = warm / rowScale with inequality multipliers clamped at 0.
+    std::vector<double> x0((size_t)h.n, 0.0), y0((size_t)h.m, 0.0);
+    const prima_warm* w = J.warm;
+    if (w && w->primal)
+      for (int32_t i = 0; i < h.n; ++i) x0[i] = w->primal[i] / h.cs[i];
+    for (int32_t i = 0; i < h.n; ++i) {
+      const double v = x0[i];
+      x0[i] = v < h.sl[i] ? h.sl[i] : (v > h.su[i] ? h.su[i] : v);
+    }
+    if (w && w->dual)
+      for (int32_t r = 0; r < h.m; ++r) {
+        const double v = w->dual[r] / h.rs[r];
+        y0[r] = (r >= h.n_eq && v < 0.0) ? 0.0 : v;
+      }
+    HIP_TRY(ctx, put_t(L.x, x0.data(), h.n));
+    HIP_TRY(ctx, put_t(L.y, y0.data(), h.m));
+
+    // initialise: step from the spectral bound, weight from data norms
+    // (Pdhg.scala:250-263); warm values override when finite and positive.
+    SolveState st;
+    std::memset(&st, 0, sizeof(st));
+    const bool ws = w && std::isfinite(w->step_size) && w->step_size > 0.0;
+    const bool ww = w && std::isfinite(w->primal_weight) && w->primal_weight > 0.0;
+    st.step = ws ? w->step_size : (h.spec_scaled > 0.0 ? 1.0 / h.spec_scaled : 1.0);
+    const double cn = norm2(h.sc.data(), h.n), qn = norm2(h.sq.data(), h.m);
+    st.weight = ww ? w->primal_weight : ((cn > 0.0 && qn > 0.0) ? cn / qn : 1.0);
+    st.err_at_restart = INFINITY;
+    st.last_candidate = INFINITY;
+    st.status = -2;
+    HIP_TRY(ctx, put(L.st, &st, sizeof(st)));
+
+    LpView v;
+    std::memset(&v, 0, sizeof(v));
+    char* a = W.arena;
+    v.n = h.n;
+    v.m = h.m;
+    v.n_eq = h.n_eq;
+    v.G = h.G;
+    v.wg_base = wg_base;
+    v.a_ptr = (const int32_t*)(a + L.a_ptr);
+    v.a_col = (const int32_t*)(a + L.a_col);
+    v.a_val = a + L.a_val;
+    v.t_ptr = (const int32_t*)(a + L.t_ptr);
+    v.t_col = (const int32_t*)(a + L.t_col);
+    v.t_val = a + L.t_val;
+    v.a_rb = (const int32_t*)(a + L.a_rb);
+    v.t_rb = (const int32_t*)(a + L.t_rb);
+    v.a_wrb = (const int32_t*)(a + L.a_wrb);
+    v.t_wrb = (const int32_t*)(a + L.t_wrb);
+    v.c = a + L.c;
+    v.q = a + L.q;
+    v.l = a + L.l;
+    v.u = a + L.u;
+    v.oc = (const double*)(a + L.oc);
+    v.oq = (const double*)(a + L.oq);
+    v.ol = (const double*)(a + L.ol);
+    v.ou = (const double*)(a + L.ou);
+    v.rs = (const double*)(a + L.rs);
+    v.cs = (const double*)(a + L.cs);
+    v.x = a + L.x;
+    v.y = a + L.y;
+    v.kx = a + L.kx;
+    v.kty = a + L.kty;
+    v.xn = a + L.xn;
+    v.yn = a + L.yn;
+    v.xbar = a + L.xbar;
+    v.kxbar = a + L.kxbar;
+    v.sx = a + L.sx;
+    v.sy = a + L.sy;
+    v.skx = a + L.skx;
+    v.skty = a + L.skty;
+    v.xr = a + L.xr;
+    v.yr = a + L.yr;
+    v.x_out = (double*)(a + L.x_out);
+    v.y_out = (double*)(a + L.y_out);
+    v.rc_out = (double*)(a + L.rc_out);
+    v.part = (double*)(a + L.part);
+    v.bar = (uint32_t*)(a + L.bar);
+    v.st = (SolveState*)(a + L.st);
+    v.offset = lp->offset;
+    v.rhs_norm = h.rhs_norm;
+    v.obj_norm = h.obj_norm;
+    v.spec_orig = h.spec_orig;
+    W.views.push_back(v);
+    for (int g = 0; g < h.G; ++g) W.wg_lp[wg_base + g] = (int32_t)k;
+    wg_base += h.G;
+  }
+  HIP_TRY(ctx, hipMalloc((void**)&W.d_views, W.views.size() * sizeof(LpView)));
+  HIP_TRY(ctx, hipMalloc((void**)&W.d_wg_lp, W.wg_lp.size() * sizeof(int32_t)));
+  HIP_TRY(ctx, hipMemcpy(W.d_views, W.views.data(), W.views.size() * sizeof(LpView),
+                         hipMemcpyHostToDevice));
+  HIP_TRY(ctx, hipMemcpy(W.d_wg_lp, W.wg_lp.data(), W.wg_lp.size() * sizeof(int32_t),
+                         hipMemcpyHostToDevice));
+  const double upload_ms = ms_since(t_up);
+
+  // chunked launches until every LP of the wave reached a status
+  KParams P;
+  std::memset(&P, 0, sizeof(P));
+  P.eps_abs = p->eps_abs;
+  P.eps_rel = p->eps_rel;
+  P.infeas_tol = p->infeasibility_tolerance;
+  P.r_suff = p->restart_sufficient;
+  P.r_nec = p->restart_necessary;
+  P.r_art = p->restart_artificial;
+  P.theta = p->primal_weight_smoothing;
+  P.max_iter = p->max_iterations;
+  P.eval_freq = p->evaluation_frequency;
+  P.mode = 0;
+  int budget = p->chunk_iterations > 0 ? p->chunk_iterations : 1024;
+  const size_t nlp = idx.size();
+  std::vector<SolveState> states(nlp);
+  auto t_solve = clk::now();
+  for (;;) {
+    P.iter_budget = budget;
+    auto t_l = clk::now();
+    HIP_TRY(ctx, launch_pdhg(prec, W.d_views, W.d_wg_lp, W.grid, P, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    const double launch_ms = ms_since(t_l);
+    for (size_t k = 0; k < nlp; ++k)
+      HIP_TRY(ctx, hipMemcpy(&states[k], W.arena + W.lay[k].st, sizeof(SolveState),
+                             hipMemcpyDeviceToHost));
+    bool all_done = true;
+    for (auto& st : states) all_done &= st.status >= 0;
+    if (all_done) break;
+    int final_status = -1;
+    if (ctx->cancel.load()) final_status = PRIMA_INTERRUPTED;
+    else if (p->time_limit_ms >= 0 && ms_since(t_solve) >= (double)p->time_limit_ms)
+      final_status = PRIMA_TIME_LIMIT;
+    if (final_status >= 0) {
+      KParams F = P;
+      F.mode = 1;
+      F.final_status = final_status;
+      HIP_TRY(ctx, launch_pdhg(prec, W.d_views, W.d_wg_lp, W.grid, F, ctx->stream));
+      HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+      for (size_t k = 0; k < nlp; ++k)
+        HIP_TRY(ctx, hipMemcpy(&states[k], W.arena + W.lay[k].st, sizeof(SolveState),
+                               hipMemcpyDeviceToHost));
+      break;
+    }
+    if (p->chunk_iterations == 0) {
+      // aim for launches of roughly 50 ms (shorter under a time limit)
+      double target = 50.0;
+      if (p->time_limit_ms >= 0) target = std::min(target, std::max(1.0, p->time_limit_ms / 8.0));
+      const double per_it = launch_ms / std::max(budget, 1);
+      int nb = (int)std::min(1e7, std::max((double)p->evaluation_frequency, target / per_it));
+      budget = std::max(nb, p->evaluation_frequency);
+    }
+  }
+  const double solve_ms = ms_since(t_solve);
+  for (size_t k = 0; k < nlp; ++k) {
+    SolveJob& J = jobs[idx[k]];
+    const LpLayout& L = W.lay[k];
+    const SolveState& st = states[k];
+    if (st.status < 0) return fail(ctx, PRIMA_EDEVICE, "solve ended without a status");
+    if (J.x_out && J.h.n)
+      HIP_TRY(ctx, hipMemcpy(J.x_out, W.arena + L.x_out, (size_t)J.h.n * 8,
+                             hipMemcpyDeviceToHost));
+    if (J.rc_out && J.h.n)
+      HIP_TRY(ctx, hipMemcpy(J.rc_out, W.arena + L.rc_out, (size_t)J.h.n * 8,
+                             hipMemcpyDeviceToHost));
+    if (J.y_out && J.h.m)
+      HIP_TRY(ctx, hipMemcpy(J.y_out, W.arena + L.y_out, (size_t)J.h.m * 8,
+                             hipMemcpyDeviceToHost));
+    prima_result* r = J.res;
+    r->status = st.status;
+    r->iterations = st.iteration;
+    r->restarts = st.restarts;
+    r->used_average = st.used_average;
+    r->primal_residual = st.kkt[0];
+    r->dual_residual = st.kkt[1];
+    r->objective = st.kkt[2];
+    r->dual_objective = st.kkt[3];
+    r->solve_time_ms = solve_ms;
+    r->total_time_ms = solve_ms + upload_ms + J.prep_ms;
+    r->final_step_size = st.step;
+    r->final_primal_weight = st.weight;
+    r->trials = st.total_trials;
+    r->evaluations = st.evals;
+    r->workgroups = J.h.G;
+  }
+  return PRIMA_OK;
+}
+
+// Solve a set of prepared jobs in waves that fit the device's co-resident
+// workgroup capacity.
+int run_jobs(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const prima_params* p) {
+  const int occ = ctx->occ[prec == 32 ? 0 : 1];
+  const int capacity = std::max(1, ctx->cu_count * occ);
+  for (auto& J : jobs) J.h.G = choose_groups(ctx, p, J.h.nnz, capacity);
+  std::vector<int> wave;
+  int used = 0;
+  for (size_t j = 0; j < jobs.size(); ++j) {
+    if (used + jobs[j].h.G > capacity && !wave.empty()) {
+      int rc = run_wave(ctx, prec, jobs, wave, p);
+      if (rc) return rc;
+      wave.clear();
+      used = 0;
+    }
+    wave.push_back((int)j);
+    used += jobs[j].h.G;
+  }
+  if (!wave.empty()) return run_wave(ctx, prec, jobs, wave, p);
+  return PRIMA_OK;
+}
+
+int prepare_job(prima_ctx* ctx, SolveJob& J, const prima_lp* lp, const prima_params* p,
+                const prima_warm* warm) {
+  std::string e = validate_lp(lp);
+  if (!e.empty()) return fail(ctx, PRIMA_EINVAL, e);
+  if (warm) {
+    // Pdhg.scala:272-293: warm vectors must be finite
+    if (warm->primal)
+      for (int32_t i = 0; i < lp->n; ++i)
+        if (!std::isfinite(warm->primal[i]))
+          return fail(ctx, PRIMA_EINVAL, "warm start primal vector is not finite");
+    if (warm->dual)
+      for (int32_t r = 0; r < lp->m; ++r)
+        if (!std::isfinite(warm->dual[r]))
+          return fail(ctx, PRIMA_EINVAL, "warm start dual vector is not finite");
+  }
+  auto t0 = clk::now();
+  J.lp = lp;
+  J.warm = warm;
+  prepare(lp, p, J.h);
+  J.prep_ms = ms_since(t0);
+  return PRIMA_OK;
+}
+
+prima_params reduced_params(const prima_params* p, const prima_mixed_params* mx) {
+  // MixedPrecision.scala:115-130
+  prima_params r = *p;
+  r.eps_abs = std::max(p->eps_abs, mx->reduced_tolerance);
+  r.eps_rel = std::max(p->eps_rel, mx->reduced_tolerance);
+  r.max_iterations = std::min(p->max_iterations, mx->reduced_max_iterations);
+  if (p->time_limit_ms >= 0)
+    r.time_limit_ms = std::max<int64_t>(1, (int64_t)(p->time_limit_ms * mx->reduced_time_share));
+  r.infeasibility_tolerance = std::max(p->infeasibility_tolerance, mx->reduced_tolerance);
+  return r;
+}
+
+std::string validate_mixed(const prima_mixed_params* mx) {
+  if (!(mx->reduced_tolerance > 0.0)) return "reducedTolerance must be positive";
+  if (mx->reduced_max_iterations <= 0) return "reducedMaxIterations must be positive";
+  if (!(mx->reduced_time_share > 0.0 && mx->reduced_time_share < 1.0))
+    return "reducedTimeShare must lie in (0, 1)";
+  return "";
+}
+
+bool conclusive(int s) {
+  return s == PRIMA_OPTIMAL || s == PRIMA_PRIMAL_INFEASIBLE || s == PRIMA_DUAL_INFEASIBLE;
+}
+
+// The two-pass contract of MixedPrecision.solve (MixedPrecision.scala:114-199)
+// for a set of LPs: float32 device pass for all, then a float64 device pass
+// warm-started from each usable reduced result that is not a confirmed
+// certificate.
+int mixed_batch(prima_ctx* ctx, int32_t count, const prima_lp* lps, const prima_params* p,
+                const prima_mixed_params* mx, const prima_warm* warms, prima_result* results,
+                prima_mixed_info* infos, double** xo, double** yo, double** rco) {
+  const prima_params rp = reduced_params(p, mx);
+  std::vector<std::vector<double>> rx(count), ry(count), rrc(count);
+  std::vector<prima_result> red(count);
+  {
+    std::vector<SolveJob> jobs(count);
+    for (int32_t k = 0; k < count; ++k) {
+      int rc = prepare_job(ctx, jobs[k], &lps[k], &rp, warms ? &warms[k] : nullptr);
+      if (rc) return rc;
+      rx[k].assign((size_t)lps[k].n, 0.0);
+      ry[k].assign((size_t)lps[k].m, 0.0);
+      rrc[k].assign((size_t)lps[k].n, 0.0);
+      jobs[k].res = &red[k];
+      jobs[k].x_out = rx[k].data();
+      jobs[k].y_out = ry[k].data();
+      jobs[k].rc_out = rrc[k].data();
+    }
+    int rc = run_jobs(ctx, 32, jobs, &rp);
+    if (rc) return rc;
+  }
+  std::vector<SolveJob> refine;
+  std::vector<prima_warm> rw(count);
+  std::vector<int> which;
+  for (int32_t k = 0; k < count; ++k) {
+    prima_mixed_info& info = infos[k];
+    std::memset(&info, 0, sizeof(info));
+    info.reduced_iterations = red[k].iterations;
+    info.reduced_restarts = red[k].restarts;
+    info.reduced_ms = red[k].solve_time_ms;
+    const int s = red[k].status;
+    const bool holds = conclusive(s) && s != PRIMA_OPTIMAL &&
+                       host_classify(&lps[k], rx[k].data(), ry[k].data(),
+                                     p->infeasibility_tolerance) == s;
+    if (!mx->always_refine && holds) {
+      results[k] = red[k];
+      if (xo && xo[k]) std::memcpy(xo[k], rx[k].data(), rx[k].size() * 8);
+      if (yo && yo[k]) std::memcpy(yo[k], ry[k].data(), ry[k].size() * 8);
+      if (rco && rco[k]) std::memcpy(rco[k], rrc[k].data(), rrc[k].size() * 8);
+      continue;
+    }
+    bool usable = true;
+    for (double v : rx[k]) usable &= std::isfinite(v);
+    for (double v : ry[k]) usable &= std::isfinite(v);
+    if (usable) {
+      rw[k].primal = rx[k].data();
+      rw[k].dual = ry[k].data();
+      rw[k].step_size = red[k].final_step_size;
+      rw[k].primal_weight = red[k].final_primal_weight;
+    } else {
+      rw[k] = prima_warm{nullptr, nullptr, NAN, NAN};
+    }
+    which.push_back(k);
+  }
+  if (which.empty()) return PRIMA_OK;
+  prima_params fp = *p;
+  std::vector<SolveJob> jobs(which.size());
+  for (size_t j = 0; j < which.size(); ++j) {
+    const int k = which[j];
+    if (p->time_limit_ms >= 0)
+      fp.time_limit_ms = std::max<int64_t>(1, p->time_limit_ms - (int64_t)red[k].solve_time_ms);
+    int rc = prepare_job(ctx, jobs[j], &lps[k], &fp, &rw[k]);
+    if (rc) return rc;
+    jobs[j].res = &results[k];
+    jobs[j].x_out = xo ? xo[k] : nullptr;
+    jobs[j].y_out = yo ? yo[k] : nullptr;
+    jobs[j].rc_out = rco ? rco[k] : nullptr;
+  }
+  int rc = run_jobs(ctx, 64, jobs, &fp);
+  if (rc) return rc;
+  for (size_t j = 0; j < which.size(); ++j) {
+    const int k = which[j];
+    infos[k].refinement_iterations = results[k].iterations;
+    infos[k].refinement_ms = results[k].solve_time_ms;
+    infos[k].refined = 1;
+    results[k].total_time_ms += red[k].total_time_ms;
+  }
+  return PRIMA_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+
+extern "C" {
+
+void prima_default_params(prima_params* p) {
+  std::memset(p, 0, sizeof(*p));
+  p->eps_abs = 1e-8;
+  p->eps_rel = 1e-8;
+  p->max_iterations = 100000;
+  p->evaluation_frequency = 64;
+  p->time_limit_ms = -1;
+  p->ruiz_iterations = 10;
+  p->apply_pock_chambolle = 1;
+  p->pock_chambolle_alpha = 1.0;
+  p->infeasibility_tolerance = 1e-8;
+  p->restart_sufficient = 0.2;
+  p->restart_necessary = 0.8;
+  p->restart_artificial = 0.36;
+  p->primal_weight_smoothing = 0.5;
+  p->workgroups_per_lp = 0;
+  p->chunk_iterations = 0;
+}
+
+void prima_default_mixed_params(prima_mixed_params* p) {
+  p->reduced_tolerance = 1e-5;
+  p->reduced_max_iterations = 20000;
+  p->always_refine = 0;
+  p->reduced_time_share = 0.5;
+}
+
+int prima_open(int32_t device, int32_t precision, prima_ctx** out) {
+  if (!out) return PRIMA_EINVAL;
+  *out = nullptr;
+  if (precision != 64 && precision != 32) return PRIMA_EINVAL;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count)
+    return PRIMA_EDEVICE;
+  if (hipSetDevice(device) != hipSuccess) return PRIMA_EDEVICE;
+  prima_ctx* ctx = new prima_ctx();
+  ctx->device = device;
+  ctx->precision = precision;
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return PRIMA_EDEVICE;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+    ctx->cu_count = prop.multiProcessorCount;
+    ctx->device_name = std::string(prop.name) + " (" + prop.gcnArchName + ")";
+  }
+  int coop = 0;
+  (void)hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device);
+  ctx->occ[0] = occupancy_blocks_per_cu(32);
+  ctx->occ[1] = occupancy_blocks_per_cu(64);
+  if (hipMalloc((void**)&ctx->scratch, 2048 * sizeof(double)) != hipSuccess) {
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return PRIMA_ENOMEM;
+  }
+  ctx->coop_ok = coop && ctx->occ[0] > 0 && ctx->occ[1] > 0 && ctx->cu_count > 0;
+  *out = ctx;
+  return PRIMA_OK;
+}
+
+int prima_close(prima_ctx* ctx) {
+  if (!ctx) return PRIMA_EINVAL;
+  if (ctx->closed) return PRIMA_OK;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->scratch) (void)hipFree(ctx->scratch);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  ctx->scratch = nullptr;
+  ctx->stream = nullptr;
+  ctx->closed = true;
+  return PRIMA_OK;
+}
+
+void prima_destroy(prima_ctx* ctx) {
+  if (!ctx) return;
+  prima_close(ctx);
+  delete ctx;
+}
+
+int prima_capabilities_of(const prima_ctx* ctx, prima_capabilities* out) {
+  if (!ctx || !out) return PRIMA_EINVAL;
+  std::memset(out, 0, sizeof(*out));
+  std::snprintf(out->name, sizeof(out->name), "%s",
+                ctx->precision == 64 ? "prima-hip-f64" : "prima-hip-f32");
+  std::snprintf(out->device, sizeof(out->device), "%s", ctx->device_name.c_str());
+  out->supports_float64 = ctx->precision == 64 ? 1 : 0;
+  return PRIMA_OK;
+}
+
+const char* prima_last_error(const prima_ctx* ctx) { return ctx ? ctx->err.c_str() : ""; }
+
+int prima_cancel(prima_ctx* ctx) {
+  if (!ctx) return PRIMA_EINVAL;
+  ctx->cancel.store(1);
+  return PRIMA_OK;
+}
+
+int prima_vec_alloc(prima_ctx* ctx, int64_t n, prima_vec** out) {
+  int rc = check_open(ctx);
+  if (rc) return rc;
+  if (!out || n < 0) return fail(ctx, PRIMA_EINVAL, "bad vector length");
+  prima_vec* v = new prima_vec();
+  v->n = n;
+  const size_t bytes = std::max<size_t>(1, (size_t)n * esize(ctx->precision));
+  if (hipMalloc(&v->d, bytes) != hipSuccess) {
+    delete v;
+    return fail(ctx, PRIMA_ENOMEM, "vector allocation");
+  }
+  hipError_t e = hipMemsetAsync(v->d, 0, bytes, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    (void)hipFree(v->d);
+    delete v;
+    return dev_fail(ctx, e, "zero-fill");
+  }
+  *out = v;
+  return PRIMA_OK;
+}
+
+int prima_vec_upload(prima_ctx* ctx, const double* data, int64_t n, prima_vec** out) {
+  int rc = prima_vec_alloc(ctx, n, out);
+  if (rc) return rc;
+  if (n == 0) return PRIMA_OK;
+  if (!data) return fail(ctx, PRIMA_EINVAL, "null data");
+  std::vector<char> tmp((size_t)n * esize(ctx->precision));
+  to_prec(ctx->precision, data, n, tmp.data());
+  HIP_TRY(ctx, hipMemcpy((*out)->d, tmp.data(), tmp.size(), hipMemcpyHostToDevice));
+  return PRIMA_OK;
+}
+
+int prima_vec_download(prima_ctx* ctx, const prima_vec* v, double* into, int64_t n) {
+  int rc = check_open(ctx);
+  if (rc) return rc;
+  if (!v || n != v->n) return fail(ctx, PRIMA_EINVAL, "length mismatch");
+  if (n == 0) return PRIMA_OK;
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  if (ctx->precision == 64) {
+    HIP_TRY(ctx, hipMemcpy(into, v->d, (size_t)n * 8, hipMemcpyDeviceToHost));
+  } else {
+    std::vector<float> tmp((size_t)n);
+    HIP_TRY(ctx, hipMemcpy(tmp.data(), v->d, (size_t)n * 4, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < n; ++i) into[i] = (double)tmp[i];
+  }
+  return PRIMA_OK;
+}
+
+int64_t prima_vec_length(const prima_vec* v) { return v ? v->n : -1; }
+
+int prima_vec_free(prima_ctx* ctx, prima_vec* v) {
+  if (!v) return PRIMA_OK;
+  if (ctx && !ctx->closed) {
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+  }
+  if (v->d) (void)hipFree(v->d);
+  delete v;
+  return PRIMA_OK;
+}
+
+int prima_mat_upload_csr(prima_ctx* ctx, int32_t rows, int32_t cols, int64_t nnz,
+                         const int32_t* row_ptr, const int32_t* col, const double* vals,
+                         prima_mat** out) {
+  int rc = check_open(ctx);
+  if (rc) return rc;
+  if (!out || rows < 0 || cols < 0 || nnz < 0 || nnz > INT32_MAX)
+    return fail(ctx, PRIMA_EINVAL, "bad matrix shape");
+  if (!row_ptr || (nnz > 0 && (!col || !vals))) return fail(ctx, PRIMA_EINVAL, "null CSR");
+  if (row_ptr[0] != 0 || row_ptr[rows] != nnz)
+    return fail(ctx, PRIMA_EINVAL, "row_ptr does not span nnz");
+  for (int32_t r = 0; r < rows; ++r) {
+    if (row_ptr[r + 1] < row_ptr[r]) return fail(ctx, PRIMA_EINVAL, "row_ptr not monotone");
+    for (int32_t p = row_ptr[r]; p < row_ptr[r + 1]; ++p) {
+      if (col[p] < 0 || col[p] >= cols) return fail(ctx, PRIMA_EINVAL, "column out of range");
+      if (!std::isfinite(vals[p])) return fail(ctx, PRIMA_EINVAL, "entry is not finite");
+    }
+  }
+  prima_mat* M = new prima_mat();
+  M->rows = rows;
+  M->cols = cols;
+  M->nnz = nnz;
+  const size_t s = esize(ctx->precision);
+  bool ok = hipMalloc((void**)&M->ptr, ((size_t)rows + 1) * 4) == hipSuccess &&
+            hipMalloc((void**)&M->col, std::max<size_t>(1, (size_t)nnz * 4)) == hipSuccess &&
+            hipMalloc(&M->val, std::max<size_t>(1, (size_t)nnz * s)) == hipSuccess;
+  if (!ok) {
+    prima_mat_free(ctx, M);
+    return fail(ctx, PRIMA_ENOMEM, "matrix allocation");
+  }
+  std::vector<char> tmp((size_t)nnz * s);
+  to_prec(ctx->precision, vals, nnz, tmp.data());
+  hipError_t e = hipMemcpy(M->ptr, row_ptr, ((size_t)rows + 1) * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess && nnz) e = hipMemcpy(M->col, col, (size_t)nnz * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess && nnz) e = hipMemcpy(M->val, tmp.data(), tmp.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    prima_mat_free(ctx, M);
+    return dev_fail(ctx, e, "matrix upload");
+  }
+  *out = M;
+  return PRIMA_OK;
+}
+
+int prima_mat_free(prima_ctx* ctx, prima_mat* M) {
+  if (!M) return PRIMA_OK;
+  if (ctx && !ctx->closed) {
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+  }
+  if (M->ptr) (void)hipFree(M->ptr);
+  if (M->col) (void)hipFree(M->col);
+  if (M->val) (void)hipFree(M->val);
+  delete M;
+  return PRIMA_OK;
+}
+
+int prima_spmv(prima_ctx* ctx, const prima_mat* a, const prima_vec* x, prima_vec* out) {
+  int rc = check_open(ctx);
+  if (rc) return rc;
+  if (!a || !x || !out || x->n != a->cols || out->n != a->rows)
+    return fail(ctx, PRIMA_EINVAL, "spmv shape mismatch");
+  if (x->d == out->d) return fail(ctx, PRIMA_EINVAL, "spmv output aliases its input");
+  HIP_TRY(ctx, op_spmv(ctx->precision, a->rows, a->nnz, a->ptr, a->col, a->val, x->d, out->d,
+                       ctx->stream));
+  return PRIMA_OK;
+}
+
+int prima_axpby(prima_ctx* ctx, double alpha, const prima_vec* x, double beta, const prima_vec* y,
+                prima_vec* out) {
+  int rc = check_open(ctx);
+  if (rc) return rc;
+  if (!x || !y || !out || x->n != out->n || y->n != out->n)
+    return fail(ctx, PRIMA_EINVAL, "axpby length mismatch");
+  HIP_TRY(ctx, op_axpby(ctx->precision, out->n, alpha, x->d, beta, y->d, out->d, ctx->stream));
+  return PRIMA_OK;
+}
+
+int prima_scale(prima_ctx* ctx, double alpha, const prima_vec* x, prima_vec* out) {
+  int rc = check_open(ctx);
+  if (rc) return rc;
+  if (!x || !out || x->n != out->n) return fail(ctx, PRIMA_EINVAL, "scale length mismatch");
+  HIP_TRY(ctx, op_scale(ctx->precision, out->n, alpha, x->d, out->d, ctx->stream));
+  return PRIMA_OK;
+}
+
+int prima_copy(prima_ctx* ctx, const prima_vec* src, prima_vec* dst) {
+  int rc = check_open(ctx);
+  if (rc) return rc;
+  if (!src || !dst || src->n != dst->n) return fail(ctx, PRIMA_EINVAL, "copy length mismatch");
+  if (src->n == 0 || src->d == dst->d) return PRIMA_OK;
+  HIP_TRY(ctx, hipMemcpyAsync(dst->d, src->d, (size_t)src->n * esize(ctx->precision),
+                              hipMemcpyDeviceToDevice, ctx->stream));
+  return PRIMA_OK;
+}
+
+int prima_dot(prima_ctx* ctx, const prima_vec* x, const prima_vec* y, double* out) {
+  int rc = check_open(ctx);
+  if (rc) return rc;
+  if (!x || !y || !out || x->n != y->n) return fail(ctx, PRIMA_EINVAL, "dot length mismatch");
+  if (x->n == 0) {
+    *out = 0.0;
+    return PRIMA_OK;
+  }
+  HIP_TRY(ctx, op_dot(ctx->precision, x->n, x->d, y->d, ctx->scratch, ctx->stream));
+  HIP_TRY(ctx, hipMemcpyAsync(out, ctx->scratch + 1024, sizeof(double), hipMemcpyDeviceToHost,
+                              ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return PRIMA_OK;
+}
+
+int prima_squared_norm(prima_ctx* ctx, const prima_vec* x, double* out) {
+  return prima_dot(ctx, x, x, out);
+}
+
+int prima_primal_step(prima_ctx* ctx, const prima_vec* x, const prima_vec* kty,
+                      const prima_vec* cost, const prima_vec* lower, const prima_vec* upper,
+                      double tau, prima_vec* out) {
+  int rc = check_open(ctx);
+  if (rc) return rc;
+  if (!x || !kty || !cost || !lower || !upper || !out) return fail(ctx, PRIMA_EINVAL, "null");
+  const int64_t n = out->n;
+  if (x->n != n || kty->n != n || cost->n != n || lower->n != n || upper->n != n)
+    return fail(ctx, PRIMA_EINVAL, "primalStep length mismatch");
+  HIP_TRY(ctx, op_primal(ctx->precision, n, x->d, kty->d, cost->d, lower->d, upper->d, tau,
+                         out->d, ctx->stream));
+  return PRIMA_OK;
+}
+
+int prima_dual_step(prima_ctx* ctx, const prima_vec* y, const prima_vec* kx_bar,
+                    const prima_vec* rhs, double sigma, int32_t n_eq, prima_vec* out) {
+  int rc = check_open(ctx);
+  if (rc) return rc;
+  if (!y || !kx_bar || !rhs || !out) return fail(ctx, PRIMA_EINVAL, "null");
+  const int64_t m = out->n;
+  if (y->n != m || kx_bar->n != m || rhs->n != m || n_eq < 0 || n_eq > m)
+    return fail(ctx, PRIMA_EINVAL, "dualStep shape mismatch");
+  HIP_TRY(ctx, op_dual(ctx->precision, m, y->d, kx_bar->d, rhs->d, sigma, n_eq, out->d,
+                       ctx->stream));
+  return PRIMA_OK;
+}
+
+int prima_solve_batch(prima_ctx* ctx, int32_t count, const prima_lp* lps,
+                      const prima_params* params, const prima_mixed_params* mixed,
+                      const prima_warm* warms, prima_result* results, prima_mixed_info* infos,
+                      double** x_outs, double** y_outs, double** rc_outs) {
+  int rc = check_open(ctx);
+  if (rc) return rc;
+  if (count < 0 || (count > 0 && (!lps || !results)) || !params)
+    return fail(ctx, PRIMA_EINVAL, "bad batch arguments");
+  std::string e = validate_params(params);
+  if (!e.empty()) return fail(ctx, PRIMA_EINVAL, e);
+  if (!ctx->coop_ok)
+    return fail(ctx, PRIMA_EDEVICE, "device lacks cooperative launch support for the solver");
+  ctx->cancel.store(0);
+  if (count == 0) return PRIMA_OK;
+  if (mixed) {
+    e = validate_mixed(mixed);
+    if (!e.empty()) return fail(ctx, PRIMA_EINVAL, e);
+    std::vector<prima_mixed_info> tmp(count);
+    if (ctx->precision == 64) {
+      // an fp64 device skips the reduced pass (MixedPrecision.scala:111-113)
+      rc = prima_solve_batch(ctx, count, lps, params, nullptr, warms, results, nullptr, x_outs,
+                             y_outs, rc_outs);
+      if (rc) return rc;
+      for (int32_t k = 0; k < count; ++k) {
+        std::memset(&tmp[k], 0, sizeof(prima_mixed_info));
+        tmp[k].refinement_iterations = results[k].iterations;
+        tmp[k].refinement_ms = results[k].solve_time_ms;
+      }
+    } else {
+      rc = mixed_batch(ctx, count, lps, params, mixed, warms, results, tmp.data(), x_outs, y_outs,
+                       rc_outs);
+      if (rc) return rc;
+    }
+    if (infos) std::memcpy(infos, tmp.data(), count * sizeof(prima_mixed_info));
+    return PRIMA_OK;
+  }
+  std::vector<SolveJob> jobs(count);
+  for (int32_t k = 0; k < count; ++k) {
+    rc = prepare_job(ctx, jobs[k], &lps[k], params, warms ? &warms[k] : nullptr);
+    if (rc) return rc;
+    jobs[k].res = &results[k];
+    jobs[k].x_out = x_outs ? x_outs[k] : nullptr;
+    jobs[k].y_out = y_outs ? y_outs[k] : nullptr;
+    jobs[k].rc_out = rc_outs ? rc_outs[k] : nullptr;
+  }
+  return run_jobs(ctx, ctx->precision, jobs, params);
+}
+
+int prima_solve(prima_ctx* ctx, const prima_lp* lp, const prima_params* params,
+                const prima_warm* warm, prima_result* res, double* x_out, double* y_out,
+                double* rc_out) {
+  if (!lp || !res) return ctx ? fail(ctx, PRIMA_EINVAL, "null argument") : PRIMA_EINVAL;
+  return prima_solve_batch(ctx, 1, lp, params, nullptr, warm, res, nullptr, &x_out, &y_out,
+                           &rc_out);
+}
+
+int prima_mixed_solve(prima_ctx* ctx, const prima_lp* lp, const prima_params* params,
+                      const prima_mixed_params* mixed, prima_result* res, prima_mixed_info* info,
+                      double* x_out, double* y_out, double* rc_out) {
+  if (!lp || !res || !mixed) return ctx ? fail(ctx, PRIMA_EINVAL, "null argument") : PRIMA_EINVAL;
+  return prima_solve_batch(ctx, 1, lp, params, mixed, nullptr, res, info, &x_out, &y_out,
+                           &rc_out);
+}
+
+}  // extern "C"

@@ -1,0 +1,202 @@
+// The eight numeric operations of the reference's kernel seam
+// (kernels/Kernels.scala:70-123), one HIP kernel each, in the context's
+// iteration precision T (double or float). These back the per-op C-ABI
+// entry points; the solver itself runs the fused persistent kernel instead.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+namespace prima {
+
+constexpr int OPS_NT = 256;
+
+static inline int grid_for(int64_t n) {
+  int64_t g = (n + OPS_NT - 1) / OPS_NT;
+  if (g < 1) g = 1;
+  if (g > 65536) g = 65536;
+  return (int)g;
+}
+
+// CSR SpMV (SparseMatrix.scala:76-86): a power-of-two sub-wave per row,
+// sub-wave width chosen from the mean row length; fixed reduction order.
+template <typename T, int TPR>
+__global__ __launch_bounds__(OPS_NT) void k_spmv(int rows, const int32_t* __restrict__ ptr,
+                                                 const int32_t* __restrict__ col,
+                                                 const T* __restrict__ val,
+                                                 const T* __restrict__ x, T* __restrict__ out) {
+  const int per_block = OPS_NT / TPR;
+  const int li = threadIdx.x % TPR;
+  for (int r = blockIdx.x * per_block + threadIdx.x / TPR; r < rows;
+       r += gridDim.x * per_block) {
+    T acc = T(0);
+    for (int p = ptr[r] + li; p < ptr[r + 1]; p += TPR) acc += val[p] * x[col[p]];
+#pragma unroll
+    for (int off = TPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if (li == 0) out[r] = acc;
+  }
+}
+
+template <typename T>
+__global__ void k_axpby(int64_t n, T a, const T* x, T b, const T* y, T* out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = a * x[i] + b * y[i];
+}
+
+template <typename T>
+__global__ void k_scale(int64_t n, T a, const T* x, T* out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = a * x[i];
+}
+
+// Primal step (ScalaKernels.scala:78-98): two comparisons, so infinite
+// bounds pass the candidate through.
+template <typename T>
+__global__ void k_primal(int64_t n, const T* x, const T* kty, const T* c, const T* l, const T* u,
+                         T tau, T* out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const T s = x[i] - tau * (c[i] - kty[i]);
+    out[i] = s < l[i] ? l[i] : (s > u[i] ? u[i] : s);
+  }
+}
+
+// Dual step (ScalaKernels.scala:100-118): rows below n_eq are free.
+template <typename T>
+__global__ void k_dual(int64_t m, const T* y, const T* kxb, const T* q, T sigma, int64_t n_eq,
+                       T* out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const T s = y[i] + sigma * (q[i] - kxb[i]);
+    out[i] = (i < n_eq || s > T(0)) ? s : T(0);
+  }
+}
+
+// Dot product, stage 1: per-thread accumulation in T, fixed-order tree in
+// double per block. Stage 2 sums the block partials in one block.
+template <typename T>
+__global__ __launch_bounds__(OPS_NT) void k_dot_partial(int64_t n, const T* x, const T* y,
+                                                        double* part) {
+  __shared__ double sh[OPS_NT / 64];
+  T acc = T(0);
+  for (int64_t i = blockIdx.x * (int64_t)OPS_NT + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * OPS_NT)
+    acc += x[i] * y[i];
+  double s = (double)acc;
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < OPS_NT / 64; ++w) t += sh[w];
+    part[blockIdx.x] = t;
+  }
+}
+
+__global__ __launch_bounds__(OPS_NT) void k_sum_partials(int count, const double* part,
+                                                         double* out) {
+  __shared__ double sh[OPS_NT / 64];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < count; i += OPS_NT) s += part[i];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < OPS_NT / 64; ++w) t += sh[w];
+    *out = t;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launchers (called by the C-ABI layer)
+
+template <typename T>
+static hipError_t spmv_t(int rows, int64_t nnz, const int32_t* ptr, const int32_t* col,
+                         const void* val, const void* x, void* out, hipStream_t s) {
+  if (rows == 0) return hipSuccess;
+  const double mean = (double)nnz / rows;
+  const T* v = (const T*)val;
+  const T* xv = (const T*)x;
+  T* o = (T*)out;
+  if (mean <= 2.0) {
+    k_spmv<T, 1><<<grid_for(rows), OPS_NT, 0, s>>>(rows, ptr, col, v, xv, o);
+  } else if (mean <= 8.0) {
+    k_spmv<T, 4><<<grid_for((int64_t)rows * 4), OPS_NT, 0, s>>>(rows, ptr, col, v, xv, o);
+  } else if (mean <= 32.0) {
+    k_spmv<T, 16><<<grid_for((int64_t)rows * 16), OPS_NT, 0, s>>>(rows, ptr, col, v, xv, o);
+  } else {
+    k_spmv<T, 64><<<grid_for((int64_t)rows * 64), OPS_NT, 0, s>>>(rows, ptr, col, v, xv, o);
+  }
+  return hipGetLastError();
+}
+
+hipError_t op_spmv(int prec, int rows, int64_t nnz, const int32_t* ptr, const int32_t* col,
+                   const void* val, const void* x, void* out, hipStream_t s) {
+  return prec == 32 ? spmv_t<float>(rows, nnz, ptr, col, val, x, out, s)
+                    : spmv_t<double>(rows, nnz, ptr, col, val, x, out, s);
+}
+
+hipError_t op_axpby(int prec, int64_t n, double a, const void* x, double b, const void* y,
+                    void* out, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  if (prec == 32)
+    k_axpby<float><<<grid_for(n), OPS_NT, 0, s>>>(n, (float)a, (const float*)x, (float)b,
+                                                  (const float*)y, (float*)out);
+  else
+    k_axpby<double><<<grid_for(n), OPS_NT, 0, s>>>(n, a, (const double*)x, b, (const double*)y,
+                                                   (double*)out);
+  return hipGetLastError();
+}
+
+hipError_t op_scale(int prec, int64_t n, double a, const void* x, void* out, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  if (prec == 32)
+    k_scale<float><<<grid_for(n), OPS_NT, 0, s>>>(n, (float)a, (const float*)x, (float*)out);
+  else
+    k_scale<double><<<grid_for(n), OPS_NT, 0, s>>>(n, a, (const double*)x, (double*)out);
+  return hipGetLastError();
+}
+
+hipError_t op_primal(int prec, int64_t n, const void* x, const void* kty, const void* c,
+                     const void* l, const void* u, double tau, void* out, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  if (prec == 32)
+    k_primal<float><<<grid_for(n), OPS_NT, 0, s>>>(n, (const float*)x, (const float*)kty,
+                                                   (const float*)c, (const float*)l,
+                                                   (const float*)u, (float)tau, (float*)out);
+  else
+    k_primal<double><<<grid_for(n), OPS_NT, 0, s>>>(n, (const double*)x, (const double*)kty,
+                                                    (const double*)c, (const double*)l,
+                                                    (const double*)u, tau, (double*)out);
+  return hipGetLastError();
+}
+
+hipError_t op_dual(int prec, int64_t m, const void* y, const void* kxb, const void* q,
+                   double sigma, int64_t n_eq, void* out, hipStream_t s) {
+  if (m == 0) return hipSuccess;
+  if (prec == 32)
+    k_dual<float><<<grid_for(m), OPS_NT, 0, s>>>(m, (const float*)y, (const float*)kxb,
+                                                 (const float*)q, (float)sigma, n_eq,
+                                                 (float*)out);
+  else
+    k_dual<double><<<grid_for(m), OPS_NT, 0, s>>>(m, (const double*)y, (const double*)kxb,
+                                                  (const double*)q, sigma, n_eq, (double*)out);
+  return hipGetLastError();
+}
+
+// part must hold at least 1024 doubles; result lands in part[1024].
+hipError_t op_dot(int prec, int64_t n, const void* x, const void* y, double* part,
+                  hipStream_t s) {
+  int g = grid_for(n);
+  if (g > 1024) g = 1024;
+  if (prec == 32)
+    k_dot_partial<float><<<g, OPS_NT, 0, s>>>(n, (const float*)x, (const float*)y, part);
+  else
+    k_dot_partial<double><<<g, OPS_NT, 0, s>>>(n, (const double*)x, (const double*)y, part);
+  k_sum_partials<<<1, OPS_NT, 0, s>>>(g, part, part + 1024);
+  return hipGetLastError();
+}
+
+}  // namespace prima

@@ -1,0 +1,849 @@
+// Persistent restarted-PDHG solver for CDNA4 (gfx950).
+//
+// One cooperative launch runs a chunk of the iteration for a batch of LPs.
+// Each LP is owned by a group of G workgroups; inside a group every
+// workgroup owns a contiguous, nnz-balanced range of constraint rows (rows
+// of A) and of variables (rows of the stored transpose A^T). A trial step
+// needs two group barriers:
+//
+//   phase 1 (own variables): [pending commit: kty = A^T yNext, x = xNext,
+//            running sums]  then the projected primal step, extrapolation
+//            and the |dx|^2 partial                 (Pdhg.scala:341-348)
+//   -- barrier A --
+//   phase 2 (own rows): kxBar = A xBar fused with the projected dual step,
+//            |dy|^2 and dy.Kdx partials             (Pdhg.scala:345-360)
+//   -- barrier B --
+//   decide: every workgroup reduces the partials of all G workgroups in the
+//            same fixed order, so the adaptive step-size decision
+//            (Pdhg.scala:356-380) is bit-identical everywhere without a
+//            third barrier; on acceptance the row-side commit
+//            (kx = (kxBar + kx)/2, y = yNext, sums; Pdhg.scala:373-389) is
+//            local to each workgroup's own rows.
+//
+// Every evaluationFrequency accepted iterations the checkpoint of
+// Pdhg.scala:396-417 runs on the device: a fresh K x, the KKT error of the
+// current and averaged iterates in original units (Kkt.scala:68-125), the
+// Farkas certificates (Certificates.scala:60-219) and the restart rule
+// (Pdhg.scala:463-502), all from one fused pass plus one reduction.
+//
+// No atomics touch numeric data: SpMV runs over an explicit transpose and
+// all sums are fixed-order trees, so results are reproducible run to run.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "pdhg_shared.h"
+
+namespace prima {
+
+constexpr int NT = 512;      // threads per workgroup
+constexpr int NWAVE = NT / 64;
+constexpr int TILE = 4096;   // nnz of a streamed row block (staged in LDS)
+
+enum {
+  E_PSQ_C = 0, E_QY_C, E_YN_C, E_CONE_C, E_ROW_C, E_DYR_C,
+  E_PSQ_A, E_QY_A, E_DYR_A,
+  E_CX_C, E_DSQ_C, E_DB_C, E_CRSQ_C, E_CVAL_C, E_XN_C, E_BSQ_C, E_DXR_C,
+  E_CX_A, E_DSQ_A, E_DB_A, E_DXR_A,
+  E_COUNT
+};
+static_assert(E_COUNT <= kEvalParts, "eval partial slots");
+
+// ---------------------------------------------------------------------------
+// reductions
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Sum K per-thread values over the workgroup; every thread receives the
+// totals. Fixed tree order: deterministic.
+template <int K>
+__device__ inline void block_sum(double (&v)[K], double* red) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double s = wave_sum(v[k]);
+    if (lane == 0) red[wv * K + k] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < K) {
+    double s = 0.0;
+    for (int w = 0; w < NWAVE; ++w) s += red[w * K + threadIdx.x];
+    red[NWAVE * K + threadIdx.x] = s;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = red[NWAVE * K + k];
+  __syncthreads();
+}
+
+// Sum K partials over the G workgroups of a group, identically in each.
+template <int K>
+__device__ inline void group_sum(const double* part, int stride, int G, double (&out)[K],
+                                 double* red) {
+  if (threadIdx.x < 64) {
+    double s[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) s[k] = 0.0;
+    for (int r = threadIdx.x; r < G; r += 64) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) s[k] += part[r * stride + k];
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) s[k] = wave_sum(s[k]);
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) red[k] = s[k];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) out[k] = red[k];
+  __syncthreads();
+}
+
+// Group barrier over G co-resident workgroups (cooperative launch). The
+// waits are bounded: a workgroup that spins too long raises the abort word,
+// which every other waiter also watches, so no launch can hang.
+__device__ inline bool group_barrier(uint32_t* bar, int G) {
+  __syncthreads();
+  if (G == 1) return true;
+  __shared__ int ok_sh;
+  if (threadIdx.x == 0) {
+    int ok = 1;
+    __threadfence();
+    uint32_t gen = __hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t arrived = __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+    if (arrived == (uint32_t)(G - 1)) {
+      __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&bar[1], gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      uint64_t spins = 0;
+      while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+        __builtin_amdgcn_s_sleep(2);
+        if ((++spins & 1023u) == 0) {
+          if (__hip_atomic_load(&bar[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+            ok = 0;
+            break;
+          }
+          if (spins > (1ull << 26)) {
+            __hip_atomic_store(&bar[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            ok = 0;
+            break;
+          }
+        }
+      }
+    }
+    __threadfence();
+    ok_sh = ok;
+  }
+  __syncthreads();
+  return ok_sh != 0;
+}
+
+// ---------------------------------------------------------------------------
+// CSR SpMV over a workgroup's row blocks (CSR-Adaptive style): a block of
+// short rows streams its products into LDS with coalesced loads and each row
+// is then summed by a power-of-two sub-wave; a block holding one row longer
+// than the tile is summed by the whole workgroup. The epilogue runs once per
+// row with the row's sum.
+template <typename T, typename Epi>
+__device__ inline void spmv_blocks(const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                                   const T* __restrict__ val, const T* __restrict__ xv,
+                                   const int32_t* __restrict__ rb, int b0, int b1, T* tile,
+                                   double* red, Epi& epi) {
+  for (int b = b0; b < b1; ++b) {
+    const int r0 = rb[b], r1 = rb[b + 1];
+    const int p0 = ptr[r0], p1 = ptr[r1];
+    const int nnz = p1 - p0, nrows = r1 - r0;
+    if (nnz > TILE) {
+      T acc = T(0);
+      for (int p = p0 + (int)threadIdx.x; p < p1; p += NT) acc += val[p] * xv[col[p]];
+      // fixed-order workgroup tree in the iteration precision
+      for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+      T* tred = tile;
+      if ((threadIdx.x & 63) == 0) tred[threadIdx.x >> 6] = acc;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        T s = T(0);
+        for (int w = 0; w < NWAVE; ++w) s += tred[w];
+        epi(r0, s);
+      }
+      __syncthreads();
+    } else {
+      int k = threadIdx.x;
+      for (; k + 3 * NT < nnz; k += 4 * NT) {
+        const int c0 = col[p0 + k], c1 = col[p0 + k + NT], c2 = col[p0 + k + 2 * NT],
+                  c3 = col[p0 + k + 3 * NT];
+        const T v0 = val[p0 + k], v1 = val[p0 + k + NT], v2 = val[p0 + k + 2 * NT],
+                v3 = val[p0 + k + 3 * NT];
+        const T x0 = xv[c0], x1 = xv[c1], x2 = xv[c2], x3 = xv[c3];
+        tile[k] = v0 * x0;
+        tile[k + NT] = v1 * x1;
+        tile[k + 2 * NT] = v2 * x2;
+        tile[k + 3 * NT] = v3 * x3;
+      }
+      for (; k < nnz; k += NT) tile[k] = val[p0 + k] * xv[col[p0 + k]];
+      __syncthreads();
+      int tpr = 1;
+      if (nrows < NT) {
+        tpr = NT / (nrows > 0 ? nrows : 1);
+        tpr = 1 << (31 - __clz(tpr));
+        if (tpr > 64) tpr = 64;
+      }
+      const int sub = threadIdx.x / tpr, li = threadIdx.x & (tpr - 1);
+      const int stride = NT / tpr;
+      for (int r = r0 + sub; r < r1; r += stride) {
+        const int s = ptr[r] - p0, e = ptr[r + 1] - p0;
+        T acc = T(0);
+        for (int q = s + li; q < e; q += tpr) acc += tile[q];
+        for (int off = tpr >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+        if (li == 0) epi(r, acc);
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Typed view of an LP: reinterprets the iteration-precision arrays.
+
+template <typename T>
+struct Lp {
+  const LpView& v;
+  __device__ Lp(const LpView& view) : v(view) {}
+  __device__ const T* aval() const { return (const T*)v.a_val; }
+  __device__ const T* tval() const { return (const T*)v.t_val; }
+  __device__ const T* c() const { return (const T*)v.c; }
+  __device__ const T* q() const { return (const T*)v.q; }
+  __device__ const T* l() const { return (const T*)v.l; }
+  __device__ const T* u() const { return (const T*)v.u; }
+  __device__ T* x() const { return (T*)v.x; }
+  __device__ T* y() const { return (T*)v.y; }
+  __device__ T* kx() const { return (T*)v.kx; }
+  __device__ T* kty() const { return (T*)v.kty; }
+  __device__ T* xn() const { return (T*)v.xn; }
+  __device__ T* yn() const { return (T*)v.yn; }
+  __device__ T* xbar() const { return (T*)v.xbar; }
+  __device__ T* kxbar() const { return (T*)v.kxbar; }
+  __device__ T* sx() const { return (T*)v.sx; }
+  __device__ T* sy() const { return (T*)v.sy; }
+  __device__ T* skx() const { return (T*)v.skx; }
+  __device__ T* skty() const { return (T*)v.skty; }
+  __device__ T* xr() const { return (T*)v.xr; }
+  __device__ T* yr() const { return (T*)v.yr; }
+};
+
+// Projected primal step, extrapolation and |dx|^2 (ScalaKernels.scala:78-98,
+// Pdhg.scala:341-348); x_i and kty_i are the committed values.
+template <typename T>
+__device__ inline void primal_update(const Lp<T>& L, int i, T xi, T ktyi, T tau, T& dx2) {
+  const T st = xi - tau * (L.c()[i] - ktyi);
+  const T li = L.l()[i], ui = L.u()[i];
+  const T xnew = st < li ? li : (st > ui ? ui : st);
+  L.xn()[i] = xnew;
+  L.xbar()[i] = T(2) * xnew - xi;
+  const T d = xnew - xi;
+  dx2 += d * d;
+}
+
+template <typename T>
+struct EpiCommitPrimal {  // rows of A^T: pending column-side commit, then primal step
+  const Lp<T>& L;
+  T w, tau;
+  bool primal;
+  T dx2;
+  __device__ void operator()(int i, T s) {
+    const T xi = L.xn()[i];
+    L.x()[i] = xi;
+    L.kty()[i] = s;
+    L.sx()[i] = w * xi + L.sx()[i];
+    L.skty()[i] = w * s + L.skty()[i];
+    if (primal) primal_update(L, i, xi, s, tau, dx2);
+  }
+};
+
+template <typename T>
+struct EpiDual {  // rows of A: kxBar and the projected dual step (ScalaKernels.scala:100-118)
+  const Lp<T>& L;
+  T sigma;
+  int n_eq;
+  T dy2, dykdx;
+  __device__ void operator()(int r, T s) {
+    const T yv = L.y()[r];
+    T ynew = yv + sigma * (L.q()[r] - s);
+    if (r >= n_eq && !(ynew > T(0))) ynew = T(0);
+    const T kdx = T(0.5) * s - T(0.5) * L.kx()[r];
+    const T d = ynew - yv;
+    L.yn()[r] = ynew;
+    L.kxbar()[r] = s;
+    dy2 += d * d;
+    dykdx += d * kdx;
+  }
+};
+
+// Row-side evaluation partials in original units (Kkt.scala:68-125,
+// Certificates.scala:60-195) for the current and, optionally, the averaged
+// iterate.
+template <typename T>
+__device__ inline void m_eval(const Lp<T>& L, int r, T kxv, bool avg, T inv, double* e) {
+  const LpView& v = L.v;
+  const double rs = v.rs[r], qv = v.oq[r];
+  const T yv = L.y()[r];
+  const double yo = (double)yv * rs, kxo = (double)kxv / rs;
+  if (r < v.n_eq) {
+    const double res = kxo - qv;
+    e[E_PSQ_C] += res * res;
+    e[E_ROW_C] += kxo * kxo;
+  } else {
+    const double sf = qv - kxo;
+    if (sf > 0.0) e[E_PSQ_C] += sf * sf;
+    if (kxo < 0.0) e[E_ROW_C] += kxo * kxo;
+    if (yo < 0.0) e[E_CONE_C] += yo * yo;
+  }
+  e[E_QY_C] += qv * yo;
+  e[E_YN_C] += yo * yo;
+  const double yrv = (double)L.yr()[r];
+  const double dyr = (double)yv - yrv;
+  e[E_DYR_C] += dyr * dyr;
+  if (avg) {
+    const T ya = inv * L.sy()[r];
+    const T ka = inv * L.skx()[r];
+    const double yao = (double)ya * rs, kao = (double)ka / rs;
+    if (r < v.n_eq) {
+      const double res = kao - qv;
+      e[E_PSQ_A] += res * res;
+    } else {
+      const double sf = qv - kao;
+      if (sf > 0.0) e[E_PSQ_A] += sf * sf;
+    }
+    e[E_QY_A] += qv * yao;
+    const double d = (double)ya - yrv;
+    e[E_DYR_A] += d * d;
+  }
+}
+
+__device__ inline void dual_terms(double rc, double lo, double hi, double& sq, double& lin) {
+  if (rc > 0.0) {
+    if (isinf(lo) && lo < 0.0) sq += rc * rc;
+    else lin += lo * rc;
+  } else if (rc < 0.0) {
+    if (isinf(hi) && hi > 0.0) sq += rc * rc;
+    else lin += hi * rc;
+  }
+}
+
+template <typename T>
+__device__ inline void n_eval(const Lp<T>& L, int i, bool avg, T inv, double* e) {
+  const LpView& v = L.v;
+  const double cs = v.cs[i], cv = v.oc[i], lo = v.ol[i], hi = v.ou[i];
+  const T xv = L.x()[i];
+  const double xo = (double)xv * cs, ko = (double)L.kty()[i] / cs;
+  e[E_CX_C] += cv * xo;
+  dual_terms(cv - ko, lo, hi, e[E_DSQ_C], e[E_DB_C]);
+  dual_terms(-ko, lo, hi, e[E_CRSQ_C], e[E_CVAL_C]);
+  e[E_XN_C] += xo * xo;
+  if (xo > 0.0 && !(isinf(hi) && hi > 0.0)) e[E_BSQ_C] += xo * xo;
+  else if (xo < 0.0 && !(isinf(lo) && lo < 0.0)) e[E_BSQ_C] += xo * xo;
+  const double xrv = (double)L.xr()[i];
+  const double dxr = (double)xv - xrv;
+  e[E_DXR_C] += dxr * dxr;
+  if (avg) {
+    const T xa = inv * L.sx()[i];
+    const T ka = inv * L.skty()[i];
+    const double xao = (double)xa * cs, kao = (double)ka / cs;
+    e[E_CX_A] += cv * xao;
+    dual_terms(cv - kao, lo, hi, e[E_DSQ_A], e[E_DB_A]);
+    const double d = (double)xa - xrv;
+    e[E_DXR_A] += d * d;
+  }
+}
+
+template <typename T>
+struct EpiFreshKx {  // rows of A: kx = A x recomputed, then row-side evaluation
+  const Lp<T>& L;
+  bool avg;
+  T inv;
+  double* e;
+  __device__ void operator()(int r, T s) {
+    L.kx()[r] = s;
+    m_eval(L, r, s, avg, inv, e);
+  }
+};
+
+template <typename T>
+struct EpiInitKty {  // rows of A^T: kty = A^T y at initialisation
+  const Lp<T>& L;
+  __device__ void operator()(int i, T s) {
+    L.kty()[i] = s;
+    L.xr()[i] = L.x()[i];
+    L.sx()[i] = T(0);
+    L.skty()[i] = T(0);
+  }
+};
+
+struct Kkt {
+  double pres, dres, pobj, dobj;
+};
+
+__device__ inline bool kkt_finite(const Kkt& e) {
+  return isfinite(e.pres) && isfinite(e.dres) && isfinite(e.pobj) && isfinite(e.dobj);
+}
+
+// Kkt.convergence (Kkt.scala:130-143) against the original data norms.
+__device__ inline bool kkt_converged(const Kkt& e, const LpView& v, const KParams& P) {
+  const double pt = P.eps_abs + P.eps_rel * v.rhs_norm;
+  const double dt = P.eps_abs + P.eps_rel * v.obj_norm;
+  const double gt = P.eps_abs + P.eps_rel * (fabs(e.pobj) + fabs(e.dobj));
+  return kkt_finite(e) && e.pres <= pt && e.dres <= dt && fabs(e.pobj - e.dobj) <= gt;
+}
+
+// KktError.weighted (Kkt.scala:28-33).
+__device__ inline double kkt_weighted(const Kkt& e, double w) {
+  const double gap = e.pobj - e.dobj;
+  return sqrt(w * e.pres * e.pres + (e.dres * e.dres) / w + gap * gap);
+}
+
+__device__ inline Kkt kkt_current(const double* E, double offset) {
+  return Kkt{sqrt(E[E_PSQ_C]), sqrt(E[E_DSQ_C]), offset + E[E_CX_C],
+             offset + E[E_QY_C] + E[E_DB_C]};
+}
+__device__ inline Kkt kkt_average(const double* E, double offset) {
+  return Kkt{sqrt(E[E_PSQ_A]), sqrt(E[E_DSQ_A]), offset + E[E_CX_A],
+             offset + E[E_QY_A] + E[E_DB_A]};
+}
+
+// Certificates.classify (Certificates.scala:207-219) on the current iterate:
+// -1 when neither certificate holds.
+__device__ inline int classify(const double* E, const LpView& v, double tol) {
+  const double spec = v.spec_orig;
+  double nsq = E[E_YN_C];
+  if (nsq > 0.0 && isfinite(nsq)) {
+    const double norm = sqrt(nsq);
+    const double value = E[E_QY_C] + E[E_CVAL_C];
+    if (value > 0.0 && isfinite(value)) {
+      const double rt = spec > 0.0 ? sqrt(E[E_CRSQ_C]) / (spec * norm) : 0.0;
+      const double ct = sqrt(E[E_CONE_C]) / norm;
+      const double h = hypot(rt, ct);
+      if (isfinite(h) && h <= tol) return 1;  // PrimalInfeasible
+    }
+  }
+  nsq = E[E_XN_C];
+  if (nsq > 0.0 && isfinite(nsq)) {
+    const double norm = sqrt(nsq);
+    const double value = E[E_CX_C];
+    if (value < 0.0 && isfinite(value)) {
+      const double rt = spec > 0.0 ? sqrt(E[E_ROW_C]) / (spec * norm) : 0.0;
+      const double bt = sqrt(E[E_BSQ_C]) / norm;
+      const double h = hypot(rt, bt);
+      if (isfinite(h) && h <= tol) return 2;  // DualInfeasible
+    }
+  }
+  return -1;
+}
+
+// ---------------------------------------------------------------------------
+// The persistent kernel. Cold paths (checkpoint evaluation, restart, result
+// write-back) are separate non-inlined functions so that their registers do
+// not burden the trial loop; reduced totals live in LDS.
+
+struct Ranges {
+  int v0, v1, r0, r1;      // own variables and rows
+  int tb0, tb1, ab0, ab1;  // own row blocks of A^T and A
+  int rank, G;
+};
+
+struct Smem {
+  double tile[TILE];
+  double red[NWAVE * kEvalParts + kEvalParts];
+  double E[kEvalParts];  // group totals of the last evaluation
+};
+
+// One evaluation pass (Kkt.evaluate + certificate inputs) over own rows and
+// variables, reduced over the group into sm.E. fresh_kx recomputes K x first
+// (Pdhg.scala:399). Returns false if a group barrier aborted.
+template <typename T>
+__device__ __noinline__ bool eval_pass(const LpView& v, const Ranges& R, bool fresh_kx, bool avg,
+                                       double wsum, Smem& sm) {
+  const Lp<T> L(v);
+  const T inv = avg ? (T)(1.0 / wsum) : T(0);
+  double e[E_COUNT];
+#pragma unroll
+  for (int k = 0; k < E_COUNT; ++k) e[k] = 0.0;
+  if (fresh_kx) {
+    EpiFreshKx<T> epi{L, avg, inv, e};
+    spmv_blocks(v.a_ptr, v.a_col, L.aval(), (const T*)L.x(), v.a_rb, R.ab0, R.ab1,
+                reinterpret_cast<T*>(sm.tile), sm.red, epi);
+  } else {
+    for (int r = R.r0 + threadIdx.x; r < R.r1; r += NT) m_eval(L, r, L.kx()[r], avg, inv, e);
+  }
+  for (int i = R.v0 + threadIdx.x; i < R.v1; i += NT) n_eval(L, i, avg, inv, e);
+  block_sum<E_COUNT>(e, sm.red);
+  double* epart = v.part + 2 * R.G * kIterParts;
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < E_COUNT; ++k) epart[R.rank * kEvalParts + k] = e[k];
+  }
+  if (!group_barrier(v.bar, R.G)) return false;
+  double tot[E_COUNT];
+  group_sum<E_COUNT>(epart, kEvalParts, R.G, tot, sm.red);
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < E_COUNT; ++k) sm.E[k] = tot[k];
+  }
+  __syncthreads();
+  return true;
+}
+
+// Result vectors in original units (Pdhg.scala:510-532): the current or
+// the averaged iterate.
+template <typename T>
+__device__ __noinline__ void write_outputs(const LpView& v, const Ranges& R, bool avg,
+                                          double wsum) {
+  const Lp<T> L(v);
+  const T inv = avg ? (T)(1.0 / wsum) : T(0);
+  for (int i = R.v0 + threadIdx.x; i < R.v1; i += NT) {
+    T xv = L.x()[i], kv = L.kty()[i];
+    if (avg) {
+      xv = inv * L.sx()[i];
+      kv = inv * L.skty()[i];
+    }
+    const double cs = v.cs[i];
+    v.x_out[i] = (double)xv * cs;
+    v.rc_out[i] = v.oc[i] - (double)kv / cs;
+  }
+  for (int r = R.r0 + threadIdx.x; r < R.r1; r += NT) {
+    T yv = L.y()[r];
+    if (avg) yv = inv * L.sy()[r];
+    v.y_out[r] = (double)yv * v.rs[r];
+  }
+}
+
+// restartAt (Pdhg.scala:478-502), the array part: re-centre on the chosen
+// iterate, record it as the restart point and clear the running sums. Every
+// array touched is in the workgroup's own range.
+template <typename T>
+__device__ __noinline__ void restart_arrays(const LpView& v, const Ranges& R, bool use_avg,
+                                           double wsum) {
+  const Lp<T> L(v);
+  const T inv = use_avg ? (T)(1.0 / wsum) : T(0);
+  for (int i = R.v0 + threadIdx.x; i < R.v1; i += NT) {
+    T xv = L.x()[i];
+    if (use_avg) {
+      xv = inv * L.sx()[i];
+      L.x()[i] = xv;
+      L.kty()[i] = inv * L.skty()[i];
+    }
+    L.xr()[i] = xv;
+    L.sx()[i] = T(0);
+    L.skty()[i] = T(0);
+  }
+  for (int r = R.r0 + threadIdx.x; r < R.r1; r += NT) {
+    T yv = L.y()[r];
+    if (use_avg) {
+      yv = inv * L.sy()[r];
+      L.y()[r] = yv;
+      L.kx()[r] = inv * L.skx()[r];
+    }
+    L.yr()[r] = yv;
+    L.sy()[r] = T(0);
+    L.skx()[r] = T(0);
+  }
+  __syncthreads();
+}
+
+// Pending column-side commit (x = xNext, kty = A^T yNext, running sums),
+// optionally followed by the next primal step; returns this thread's |dx|^2.
+template <typename T>
+__device__ inline T commit_columns(const LpView& v, const Ranges& R, T w, bool primal, T tau,
+                                   Smem& sm) {
+  const Lp<T> L(v);
+  EpiCommitPrimal<T> epi{L, w, tau, primal, T(0)};
+  spmv_blocks(v.t_ptr, v.t_col, L.tval(), (const T*)L.yn(), v.t_rb, R.tb0, R.tb1,
+              reinterpret_cast<T*>(sm.tile), sm.red, epi);
+  return epi.dx2;
+}
+
+template <typename T>
+__device__ __noinline__ void commit_columns_cold(const LpView& v, const Ranges& R, T w,
+                                                 Smem& sm) {
+  (void)commit_columns<T>(v, R, w, false, T(0), sm);
+}
+
+// Initialisation on the device (Pdhg.scala:305-309): kty = A^T y, restart
+// point, zero running sums.
+template <typename T>
+__device__ __noinline__ void init_arrays(const LpView& v, const Ranges& R, Smem& sm) {
+  const Lp<T> L(v);
+  EpiInitKty<T> epi{L};
+  spmv_blocks(v.t_ptr, v.t_col, L.tval(), (const T*)L.y(), v.t_rb, R.tb0, R.tb1,
+              reinterpret_cast<T*>(sm.tile), sm.red, epi);
+  for (int r = R.r0 + threadIdx.x; r < R.r1; r += NT) {
+    L.yr()[r] = L.y()[r];
+    L.sy()[r] = T(0);
+    L.skx()[r] = T(0);
+  }
+}
+
+__device__ __noinline__ void store_state(const LpView& v, const Ranges& R, const SolveState& S) {
+  if (R.rank == 0 && threadIdx.x == 0) *v.st = S;
+}
+
+template <typename T>
+__device__ __noinline__ void finish(const LpView& v, const Ranges& R, SolveState& S, int status,
+                                   const Kkt& err, bool avg) {
+  write_outputs<T>(v, R, avg, S.wsum);
+  S.status = status;
+  S.kkt[0] = err.pres;
+  S.kkt[1] = err.dres;
+  S.kkt[2] = err.pobj;
+  S.kkt[3] = err.dobj;
+  S.used_average = avg ? 1 : 0;
+  store_state(v, R, S);
+}
+
+__device__ __noinline__ void abort_launch(const LpView& v, const Ranges& R, SolveState& S) {
+  S.status = 5;  // NumericalError: a group barrier timed out
+  store_state(v, R, S);
+}
+
+// The checkpoint decision (Pdhg.scala:396-417, 463-502) from the group
+// totals in sm.E. Returns a terminal status (and sets *avg_out) or -1; on a
+// restart the arrays are re-centred.
+template <typename T>
+__device__ __noinline__ int checkpoint_decide(const LpView& v, const Ranges& R, const KParams& P,
+                                              SolveState& S, bool have_avg, Smem& sm,
+                                              bool* avg_out, Kkt* err_out) {
+  const double* E = sm.E;
+  const Kkt cur = kkt_current(E, v.offset);
+  *avg_out = false;
+  *err_out = cur;
+  if (kkt_converged(cur, v, P)) return 0;
+  const int cls = classify(E, v, P.infeas_tol);
+  if (cls > 0) return cls;
+  const double cur_err = kkt_weighted(cur, S.weight);
+  Kkt avg = cur;
+  double avg_err = INFINITY;
+  if (have_avg) {
+    avg = kkt_average(E, v.offset);
+    if (kkt_converged(avg, v, P)) {
+      *avg_out = true;
+      *err_out = avg;
+      return 0;
+    }
+    avg_err = kkt_weighted(avg, S.weight);
+  }
+  const bool use_avg = avg_err < cur_err;
+  const double cand = (isnan(cur_err) || isnan(avg_err)) ? NAN : fmin(cur_err, avg_err);
+  const bool sufficient = cand <= P.r_suff * S.err_at_restart;
+  const bool necessary = cand <= P.r_nec * S.err_at_restart && cand > S.last_candidate;
+  const bool artificial =
+      (double)(S.iteration - S.last_restart_it) >= P.r_art * (double)S.iteration;
+  if (sufficient || necessary || artificial) {
+    restart_arrays<T>(v, R, use_avg, S.wsum);
+    const double pm = sqrt(use_avg ? E[E_DXR_A] : E[E_DXR_C]);
+    const double dm = sqrt(use_avg ? E[E_DYR_A] : E[E_DYR_C]);
+    if (pm > 0.0 && dm > 0.0) {
+      const double nx = exp(P.theta * log(dm / pm) + (1.0 - P.theta) * log(S.weight));
+      if (isfinite(nx) && nx > 0.0) S.weight = nx;
+    }
+    S.wsum = 0.0;
+    S.restarts++;
+    S.last_restart_it = S.iteration;
+    S.err_at_restart = kkt_weighted(use_avg ? avg : cur, S.weight);
+    S.last_candidate = INFINITY;
+  } else {
+    S.last_candidate = cand;
+  }
+  return -1;
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void pdhg_persistent(const LpView* __restrict__ views,
+                                                      const int32_t* __restrict__ wg_lp,
+                                                      KParams P) {
+  __shared__ Smem sm;
+  T* tile = reinterpret_cast<T*>(sm.tile);
+
+  const LpView& v = views[wg_lp[blockIdx.x]];
+  const Lp<T> L(v);
+  Ranges R;
+  R.G = v.G;
+  R.rank = (int)blockIdx.x - v.wg_base;
+  R.tb0 = v.t_wrb[R.rank];
+  R.tb1 = v.t_wrb[R.rank + 1];
+  R.ab0 = v.a_wrb[R.rank];
+  R.ab1 = v.a_wrb[R.rank + 1];
+  R.v0 = v.t_rb[R.tb0];
+  R.v1 = v.t_rb[R.tb1];
+  R.r0 = v.a_rb[R.ab0];
+  R.r1 = v.a_rb[R.ab1];
+  const int G = R.G;
+
+  SolveState S = *v.st;
+  if (S.status >= 0) return;
+  double* ipart = v.part;  // [2][G][kIterParts]
+
+  if (P.mode == 1) {
+    if (!eval_pass<T>(v, R, false, false, S.wsum, sm)) return abort_launch(v, R, S);
+    return finish<T>(v, R, S, P.final_status, kkt_current(sm.E, v.offset), false);
+  }
+
+  if (S.status == -2) {
+    init_arrays<T>(v, R, sm);
+    if (!group_barrier(v.bar, G)) return abort_launch(v, R, S);
+    S.wsum = 0.0;
+    if (!eval_pass<T>(v, R, true, false, 0.0, sm)) return abort_launch(v, R, S);
+    const Kkt cur = kkt_current(sm.E, v.offset);
+    if (kkt_converged(cur, v, P)) return finish<T>(v, R, S, 0, cur, false);
+    S.err_at_restart = kkt_weighted(cur, S.weight);
+    S.status = -1;
+  }
+
+  const int it_start = S.iteration;
+  bool pending = false;
+  T wpend = T(0);
+
+  for (;;) {
+    const T tau = (T)(S.step / S.weight);
+    const T sigma = (T)(S.step * S.weight);
+    const int slot = S.trial_counter & 1;
+    double* ip = ipart + (slot * G + R.rank) * kIterParts;
+
+    // phase 1: own variables
+    T dx2 = T(0);
+    if (pending) {
+      dx2 = commit_columns<T>(v, R, wpend, true, tau, sm);
+      pending = false;
+    } else {
+      for (int i = R.v0 + threadIdx.x; i < R.v1; i += NT)
+        primal_update(L, i, L.x()[i], L.kty()[i], tau, dx2);
+    }
+    {
+      double a[1] = {(double)dx2};
+      block_sum<1>(a, sm.red);
+      if (threadIdx.x == 0) ip[0] = a[0];
+    }
+    if (!group_barrier(v.bar, G)) return abort_launch(v, R, S);
+
+    // phase 2: own rows
+    {
+      EpiDual<T> epi{L, sigma, v.n_eq, T(0), T(0)};
+      spmv_blocks(v.a_ptr, v.a_col, L.aval(), (const T*)L.xbar(), v.a_rb, R.ab0, R.ab1, tile,
+                  sm.red, epi);
+      double a[2] = {(double)epi.dy2, (double)epi.dykdx};
+      block_sum<2>(a, sm.red);
+      if (threadIdx.x == 0) {
+        ip[1] = a[0];
+        ip[2] = a[1];
+      }
+    }
+    if (!group_barrier(v.bar, G)) return abort_launch(v, R, S);
+
+    // decide (identical in every workgroup)
+    double tot[3];
+    group_sum<3>(ipart + slot * G * kIterParts, kIterParts, G, tot, sm.red);
+    S.trial_counter++;
+    S.trials++;
+    S.total_trials++;
+    const double movement = 0.5 * (S.weight * tot[0] + tot[1] / S.weight);
+
+    if (movement == 0.0) {
+      // fixed point (Pdhg.scala:221-231): fresh K x, then the usual test
+      S.iteration++;
+      S.trials = 0;
+      if (!eval_pass<T>(v, R, true, false, S.wsum, sm)) return abort_launch(v, R, S);
+      const Kkt cur = kkt_current(sm.E, v.offset);
+      return finish<T>(v, R, S, kkt_converged(cur, v, P) ? 0 : 5, cur, false);
+    }
+
+    const double inter = fabs(tot[2]);
+    const double limit = inter > 0.0 ? movement / inter : INFINITY;
+    const double t = (double)(S.iteration + 1);
+    const double by_rule = (1.0 - pow(t, -0.3)) * limit;
+    const double by_cap = (1.0 + pow(t, -0.6)) * S.step;
+    // math.min propagates NaN (0 * inf at the first iteration)
+    const double proposed = (isnan(by_rule) || isnan(by_cap)) ? NAN : fmin(by_rule, by_cap);
+    const bool accepted = S.step <= limit || S.trials >= 60;
+
+    if (accepted) {
+      const T w = (T)S.step;
+      for (int r = R.r0 + threadIdx.x; r < R.r1; r += NT) {
+        const T kxv = T(0.5) * L.kxbar()[r] + T(0.5) * L.kx()[r];
+        const T yv = L.yn()[r];
+        L.kx()[r] = kxv;
+        L.y()[r] = yv;
+        L.sy()[r] = w * yv + L.sy()[r];
+        L.skx()[r] = w * kxv + L.skx()[r];
+      }
+      S.wsum += S.step;
+      pending = true;
+      wpend = w;
+      S.iteration++;
+      S.trials = 0;
+    }
+    if (isfinite(proposed) && proposed > 0.0) S.step = proposed;
+    if (!accepted) continue;
+
+    bool evaluated_now = false;
+    if (S.iteration % P.eval_freq == 0) {
+      // checkpoint (Pdhg.scala:396-417)
+      commit_columns_cold<T>(v, R, wpend, sm);
+      pending = false;
+      if (!group_barrier(v.bar, G)) return abort_launch(v, R, S);
+      const bool have_avg = S.wsum > 0.0;
+      if (!eval_pass<T>(v, R, true, have_avg, S.wsum, sm)) return abort_launch(v, R, S);
+      S.evals++;
+      evaluated_now = true;
+      bool avg = false;
+      Kkt err;
+      const int st = checkpoint_decide<T>(v, R, P, S, have_avg, sm, &avg, &err);
+      if (st >= 0) return finish<T>(v, R, S, st, err, avg);
+    }
+
+    if (S.iteration >= P.max_iter) {
+      if (pending) {
+        commit_columns_cold<T>(v, R, wpend, sm);
+        pending = false;
+      }
+      __syncthreads();
+      if (!eval_pass<T>(v, R, false, false, S.wsum, sm)) return abort_launch(v, R, S);
+      return finish<T>(v, R, S, 3, kkt_current(sm.E, v.offset), false);
+    }
+
+    if (evaluated_now && S.iteration - it_start >= P.iter_budget) {
+      // end of this launch's chunk: state is fully committed
+      return store_state(v, R, S);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host-side launchers
+
+int occupancy_blocks_per_cu(int precision) {
+  int nb = 0;
+  hipError_t e;
+  if (precision == 32)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pdhg_persistent<float>, NT, 0);
+  else
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pdhg_persistent<double>, NT, 0);
+  return e == hipSuccess ? nb : 0;
+}
+
+hipError_t launch_pdhg(int precision, const LpView* views, const int32_t* wg_lp, int grid,
+                       const KParams& P, hipStream_t stream) {
+  void* args[] = {(void*)&views, (void*)&wg_lp, (void*)&P};
+  if (precision == 32)
+    return hipLaunchCooperativeKernel((const void*)pdhg_persistent<float>, dim3(grid), dim3(NT),
+                                      args, 0, stream);
+  return hipLaunchCooperativeKernel((const void*)pdhg_persistent<double>, dim3(grid), dim3(NT),
+                                    args, 0, stream);
+}
+
+int threads_per_block() { return NT; }
+int tile_nnz() { return TILE; }
+
+}  // namespace prima
