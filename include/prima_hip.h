@@ -235,6 +235,22 @@ int prima_solve_batch(prima_ctx* ctx, int32_t count, const prima_lp* lps,
                       const prima_warm* warms, prima_result* results, prima_mixed_info* infos,
                       double** x_outs, double** y_outs, double** rc_outs);
 
+/* Launch statistics of this context's solver kernel, accumulated since the
+ * last reset: launches and their total duration, timed with HIP events on
+ * the context's stream. No counterpart in the reference (its only timing is
+ * solveTimeMillis); used by the benchmark for the roofline figure. */
+typedef struct {
+  int64_t launches;
+  double kernel_ms;
+  int64_t accepted_iterations;
+  int64_t trials;
+  int64_t evaluations;
+  double algorithmic_bytes; /* compulsory HBM bytes of the work done: every array
+                             * counted once, no gather amplification */
+} prima_stats;
+int prima_stats_get(const prima_ctx* ctx, prima_stats* out);
+int prima_stats_reset(prima_ctx* ctx);
+
 /* Cooperative cancellation (the `cancelled` callback of Pdhg.scala:43-47):
  * a running solve on ctx stops at its next launch boundary with status
  * PRIMA_INTERRUPTED. Safe to call from another thread. */

@@ -52,6 +52,8 @@ struct prima_ctx {
   std::atomic<int> cancel{0};
   std::string device_name;
   bool coop_ok = false;
+  prima_stats stats{};
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
 struct prima_vec {
@@ -699,9 +701,17 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
   for (;;) {
     P.iter_budget = budget;
     auto t_l = clk::now();
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     HIP_TRY(ctx, launch_pdhg(prec, W.d_views, W.d_wg_lp, W.grid, P, ctx->stream));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     const double launch_ms = ms_since(t_l);
+    {
+      float ev_ms = 0.f;
+      HIP_TRY(ctx, hipEventElapsedTime(&ev_ms, ctx->ev0, ctx->ev1));
+      ctx->stats.kernel_ms += ev_ms;
+      ctx->stats.launches += 1;
+    }
     for (size_t k = 0; k < nlp; ++k)
       HIP_TRY(ctx, hipMemcpy(&states[k], W.arena + W.lay[k].st, sizeof(SolveState),
                              hipMemcpyDeviceToHost));
@@ -716,8 +726,16 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
       KParams F = P;
       F.mode = 1;
       F.final_status = final_status;
+      HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
       HIP_TRY(ctx, launch_pdhg(prec, W.d_views, W.d_wg_lp, W.grid, F, ctx->stream));
+      HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
       HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+      {
+        float ev_ms = 0.f;
+        HIP_TRY(ctx, hipEventElapsedTime(&ev_ms, ctx->ev0, ctx->ev1));
+        ctx->stats.kernel_ms += ev_ms;
+        ctx->stats.launches += 1;
+      }
       for (size_t k = 0; k < nlp; ++k)
         HIP_TRY(ctx, hipMemcpy(&states[k], W.arena + W.lay[k].st, sizeof(SolveState),
                                hipMemcpyDeviceToHost));
@@ -747,6 +765,21 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
     if (J.y_out && J.h.m)
       HIP_TRY(ctx, hipMemcpy(J.y_out, W.arena + L.y_out, (size_t)J.h.m * 8,
                              hipMemcpyDeviceToHost));
+    {
+      // compulsory bytes (each array counted once, no gather amplification):
+      // accepted iteration 2Z(s+4) + 4(m+n+2) + s(13n+11m); rejected trial
+      // Z(s+4) + 4(m+1) + s(n+m) + s(7n+5m); checkpoint Z(s+4) + 4(m+1)
+      // + s(n+m) + 8(4n+4m)
+      const double Z = (double)J.h.nnz, n = J.h.n, m = J.h.m, sb = (double)s;
+      const double it = st.iteration, rej = (double)(st.total_trials - st.iteration);
+      const double b_iter = 2 * Z * (sb + 4) + 4 * (m + n + 2) + sb * (13 * n + 11 * m);
+      const double b_rej = Z * (sb + 4) + 4 * (m + 1) + sb * (n + m) + sb * (7 * n + 5 * m);
+      const double b_eval = Z * (sb + 4) + 4 * (m + 1) + sb * (n + m) + 8 * (4 * n + 4 * m);
+      ctx->stats.algorithmic_bytes += it * b_iter + rej * b_rej + st.evals * b_eval;
+      ctx->stats.accepted_iterations += st.iteration;
+      ctx->stats.trials += st.total_trials;
+      ctx->stats.evaluations += st.evals;
+    }
     prima_result* r = J.res;
     r->status = st.status;
     r->iterations = st.iteration;
@@ -978,7 +1011,8 @@ int prima_open(int32_t device, int32_t precision, prima_ctx** out) {
   (void)hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device);
   ctx->occ[0] = occupancy_blocks_per_cu(32);
   ctx->occ[1] = occupancy_blocks_per_cu(64);
-  if (hipMalloc((void**)&ctx->scratch, 2048 * sizeof(double)) != hipSuccess) {
+  if (hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
+      hipMalloc((void**)&ctx->scratch, 2048 * sizeof(double)) != hipSuccess) {
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return PRIMA_ENOMEM;
@@ -994,6 +1028,9 @@ int prima_close(prima_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   if (ctx->scratch) (void)hipFree(ctx->scratch);
+  if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+  if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  ctx->ev0 = ctx->ev1 = nullptr;
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   ctx->scratch = nullptr;
   ctx->stream = nullptr;
@@ -1018,6 +1055,18 @@ int prima_capabilities_of(const prima_ctx* ctx, prima_capabilities* out) {
 }
 
 const char* prima_last_error(const prima_ctx* ctx) { return ctx ? ctx->err.c_str() : ""; }
+
+int prima_stats_get(const prima_ctx* ctx, prima_stats* out) {
+  if (!ctx || !out) return PRIMA_EINVAL;
+  *out = ctx->stats;
+  return PRIMA_OK;
+}
+
+int prima_stats_reset(prima_ctx* ctx) {
+  if (!ctx) return PRIMA_EINVAL;
+  std::memset(&ctx->stats, 0, sizeof(ctx->stats));
+  return PRIMA_OK;
+}
 
 int prima_cancel(prima_ctx* ctx) {
   if (!ctx) return PRIMA_EINVAL;

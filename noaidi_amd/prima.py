@@ -105,6 +105,12 @@ class _MixedInfo(C.Structure):
                 ("reduced_ms", C.c_double), ("refinement_ms", C.c_double)]
 
 
+class _Stats(C.Structure):
+    _fields_ = [("launches", C.c_int64), ("kernel_ms", C.c_double),
+                ("accepted_iterations", C.c_int64), ("trials", C.c_int64),
+                ("evaluations", C.c_int64), ("algorithmic_bytes", C.c_double)]
+
+
 class _Instance(C.Structure):
     _fields_ = [("n", C.c_int32), ("m", C.c_int32), ("n_eq", C.c_int32), ("pad", C.c_int32),
                 ("nnz", C.c_int64), ("row_ptr", C.POINTER(C.c_int32)),
@@ -122,7 +128,8 @@ EXPORTED = [
     "prima_vec_free", "prima_mat_upload_csr", "prima_mat_free", "prima_spmv", "prima_axpby",
     "prima_scale", "prima_copy", "prima_dot", "prima_squared_norm", "prima_primal_step",
     "prima_dual_step", "prima_default_params", "prima_default_mixed_params", "prima_solve",
-    "prima_mixed_solve", "prima_solve_batch", "prima_cancel",
+    "prima_mixed_solve", "prima_solve_batch", "prima_cancel", "prima_stats_get",
+    "prima_stats_reset",
 ]
 
 _lib = None
@@ -171,6 +178,8 @@ def lib():
                                         C.POINTER(_Mixed), C.POINTER(_Warm), C.POINTER(_Result),
                                         C.POINTER(_MixedInfo), vp, vp, vp]),
             "prima_cancel": (i32, [vp]),
+            "prima_stats_get": (i32, [vp, C.POINTER(_Stats)]),
+            "prima_stats_reset": (i32, [vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -285,6 +294,156 @@ class PdhgParams:
 
 
 LOOSE = PdhgParams(eps_abs=1e-6, eps_rel=1e-6)
+
+@dataclass
+class RowTranslation:
+    """How each original two-sided row maps to standard-form rows
+    (LpProblem.scala:110-147): ("direct", r), ("negated", r) or
+    ("range", lower_row, upper_row)."""
+
+    expansions: list
+
+    @property
+    def num_original_rows(self) -> int:
+        return len(self.expansions)
+
+    def original_duals(self, standard_duals) -> np.ndarray:
+        out = np.zeros(len(self.expansions))
+        for i, e in enumerate(self.expansions):
+            if e[0] == "direct":
+                out[i] = standard_duals[e[1]]
+            elif e[0] == "negated":
+                out[i] = -standard_duals[e[1]]
+            else:
+                out[i] = standard_duals[e[1]] - standard_duals[e[2]]
+        return out
+
+
+class LpBuilder:
+    """LpBuilder (LpProblem.scala:154-248): rows `lo <= a'x <= hi`, converted
+    to equalities first, then `>=` rows; `<=` rows are negated and ranges split
+    into two rows."""
+
+    def __init__(self, num_variables: int):
+        if num_variables < 0:
+            raise ValueError(f"numVariables must be non-negative, got {num_variables}")
+        self.n = num_variables
+        self.c = np.zeros(num_variables)
+        self.l = np.full(num_variables, -np.inf)
+        self.u = np.full(num_variables, np.inf)
+        self.offset = 0.0
+        self.rows: list = []
+
+    def _check(self, v):
+        if not (0 <= v < self.n):
+            raise ValueError(f"variable index {v} out of range [0, {self.n})")
+
+    def objectiveCoefficient(self, v: int, value: float) -> "LpBuilder":
+        self._check(v)
+        self.c[v] = value
+        return self
+
+    def objectiveOffset(self, value: float) -> "LpBuilder":
+        self.offset = value
+        return self
+
+    def bounds(self, v: int, lo: float, hi: float) -> "LpBuilder":
+        self._check(v)
+        if not lo <= hi:
+            raise ValueError(f"variable {v} has empty bound interval [{lo}, {hi}]")
+        self.l[v], self.u[v] = lo, hi
+        return self
+
+    def constraint(self, coefficients, lo: float, hi: float) -> "LpBuilder":
+        if not lo <= hi:
+            raise ValueError(f"constraint has empty range [{lo}, {hi}]")
+        if lo == -np.inf and hi == np.inf:
+            raise ValueError("constraint is unbounded on both sides and would have no effect")
+        coefficients = list(coefficients)
+        for v, _ in coefficients:
+            self._check(v)
+        self.rows.append((coefficients, lo, hi))
+        return self
+
+    def equalityConstraint(self, coefficients, value: float) -> "LpBuilder":
+        return self.constraint(coefficients, value, value)
+
+    def greaterThan(self, coefficients, lo: float) -> "LpBuilder":
+        return self.constraint(coefficients, lo, np.inf)
+
+    def lessThan(self, coefficients, hi: float) -> "LpBuilder":
+        return self.constraint(coefficients, -np.inf, hi)
+
+    def build(self):
+        trips_r, trips_c, trips_v, rhs = [], [], [], []
+        exp: list = [None] * len(self.rows)
+
+        def emit(coefs, sign, value):
+            r = len(rhs)
+            for v, a in coefs:
+                trips_r.append(r)
+                trips_c.append(v)
+                trips_v.append(sign * a)
+            rhs.append(sign * value)
+            return r
+
+        for i, (co, lo, hi) in enumerate(self.rows):
+            if lo == hi:
+                exp[i] = ("direct", emit(co, 1.0, lo))
+        n_eq = len(rhs)
+        for i, (co, lo, hi) in enumerate(self.rows):
+            if lo == hi:
+                continue
+            has_lo, has_hi = lo != -np.inf, hi != np.inf
+            if has_lo and has_hi:
+                exp[i] = ("range", emit(co, 1.0, lo), emit(co, -1.0, hi))
+            elif has_lo:
+                exp[i] = ("direct", emit(co, 1.0, lo))
+            else:
+                exp[i] = ("negated", emit(co, -1.0, hi))
+        m = len(rhs)
+        row_ptr, col, val = csr_from_triplets(m, self.n, trips_r, trips_c, trips_v)
+        p = LpProblem(self.n, m, n_eq, row_ptr, col, val, self.c.copy(), np.array(rhs, float),
+                      self.l.copy(), self.u.copy(), self.offset)
+        return p, RowTranslation(exp)
+
+
+def csr_from_triplets(rows: int, cols: int, r, c, v):
+    """SparseMatrix.fromTriplets (SparseMatrix.scala:204-260): stable sort by
+    (row, col), duplicates summed in input order, exact zeros dropped,
+    non-finite values rejected."""
+    r = np.asarray(r, dtype=np.int64)
+    c = np.asarray(c, dtype=np.int64)
+    v = np.asarray(v, dtype=np.float64)
+    if len(r) and (r.min() < 0 or r.max() >= rows or c.min() < 0 or c.max() >= cols):
+        raise ValueError("triplet index out of range")
+    if not np.all(np.isfinite(v)):
+        raise ValueError("matrix entry is not finite")
+    order = np.lexsort((c, r))  # stable
+    r, c, v = r[order], c[order], v[order]
+    keep_r, keep_c, keep_v = [], [], []
+    i = 0
+    while i < len(r):
+        j = i + 1
+        acc = v[i]
+        while j < len(r) and r[j] == r[i] and c[j] == c[i]:
+            acc += v[j]
+            j += 1
+        if acc != 0.0:
+            keep_r.append(r[i])
+            keep_c.append(c[i])
+            keep_v.append(acc)
+        i = j
+    row_ptr = np.zeros(rows + 1, dtype=np.int32)
+    np.add.at(row_ptr, np.asarray(keep_r, dtype=np.int64) + 1, 1)
+    row_ptr = np.cumsum(row_ptr, dtype=np.int64).astype(np.int32)
+    return row_ptr, np.asarray(keep_c, dtype=np.int32), np.asarray(keep_v, dtype=np.float64)
+
+
+def builder(num_variables: int) -> LpBuilder:
+    """LpProblem.builder (LpProblem.scala:105)."""
+    return LpBuilder(num_variables)
+
 
 
 @dataclass
@@ -493,6 +652,14 @@ class HipKernels:
 
     def cancel(self) -> None:
         lib().prima_cancel(self._h)
+
+    def stats(self) -> dict:
+        st = _Stats()
+        self._check(lib().prima_stats_get(self._h, C.byref(st)), "stats")
+        return {f[0]: getattr(st, f[0]) for f in _Stats._fields_}
+
+    def reset_stats(self) -> None:
+        self._check(lib().prima_stats_reset(self._h), "reset_stats")
 
     # coarse solve -------------------------------------------------------------
     def solve(self, problem: LpProblem, params: PdhgParams = PdhgParams(),

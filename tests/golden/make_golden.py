@@ -1,0 +1,50 @@
+"""Write golden objectives for the seeded dispatch instances with HiGHS
+(scipy.optimize.linprog, method="highs"), an LP oracle independent of the
+reference and of this backend. Run in a container that has scipy; the JSON
+it writes is the fixture the tests and the benchmark read.
+
+  python tests/golden/make_golden.py
+"""
+import json
+import os
+import sys
+import time
+from concurrent.futures import ProcessPoolExecutor
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+
+def highs(spec):
+    from scipy.optimize import linprog
+    from noaidi_amd import prima
+    kind, kw = spec
+    p = prima.dispatch(**kw) if kind == "dispatch" else prima.random_known(**kw)
+    A = p.to_scipy()
+    ne = p.n_eq
+    t = time.time()
+    r = linprog(p.c, A_eq=A[:ne] if ne else None, b_eq=p.q[:ne] if ne else None,
+                A_ub=-A[ne:] if p.m > ne else None, b_ub=-p.q[ne:] if p.m > ne else None,
+                bounds=list(zip(p.l, p.u)), method="highs",
+                options=dict(primal_feasibility_tolerance=1e-10, dual_feasibility_tolerance=1e-10))
+    return dict(kind=kind, args=kw, n=p.n, m=p.m, n_eq=ne, nnz=p.nnz, status=int(r.status),
+                objective=float(r.fun) if r.status == 0 else None, seconds=time.time() - t)
+
+
+def main():
+    specs = [("dispatch", dict(seed=1))]  # the 200-bus, 24-snapshot dispatch LP
+    specs += [("dispatch", dict(seed=1, perturb=0.2, scenario_seed=1000 + k)) for k in range(32)]
+    specs += [("dispatch", dict(seed=s, buses=20, chords=10, snapshots=6, storage=5))
+              for s in range(8)]
+    specs += [("dispatch", dict(seed=7, buses=60, chords=30, snapshots=24, storage=15))]
+    with ProcessPoolExecutor(8) as ex:
+        out = list(ex.map(highs, specs))
+    with open(os.path.join(HERE, "highs_dispatch.json"), "w") as f:
+        json.dump(out, f, indent=1)
+    print(len(out), "instances written")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,221 @@
+"""Parity of the on-device restarted PDHG (prima_solve / prima_mixed_solve /
+prima_solve_batch) against the oracle, the reference's fixtures and ladder,
+closed-form optima and HiGHS goldens. Gates: same status; objective within
+1e-8 relative (|a-b|/max(1,|b|), ValidationLadder.scala:45-46) at eps 1e-8
+unless stated; an infeasible instance is never Optimal."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from noaidi_amd import prima
+from oracle.oracle import OracleLp, default_params
+from tests.lp_fixtures import FIXTURES, LADDER, OPTIMAL, RANDOM_LADDER, fixture
+
+pytestmark = pytest.mark.gpu
+
+P9 = prima.PdhgParams(eps_abs=1e-9, eps_rel=1e-9, max_iterations=200_000)
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "highs_dispatch.json")
+
+
+def gap(a, b):
+    return abs(a - b) / max(1.0, abs(b))
+
+
+def oracle_of(p):
+    return OracleLp.from_csr(p.n, p.m, p.n_eq, p.row_ptr, p.col, p.val, p.c, p.q, p.l, p.u,
+                             p.offset)
+
+
+def gold(pred):
+    with open(GOLD) as f:
+        return [d for d in json.load(f) if pred(d["args"])]
+
+
+@pytest.mark.parametrize("name", list(FIXTURES))
+def test_fixtures_fp64(hip64, name):
+    # PdhgSuite.scala:12-35 at eps 1e-9, plus the ladder's counts
+    p, eobj, ex, status = fixture(name)
+    s = hip64.solve(p, P9)
+    assert s.status == status, s
+    if eobj is not None:
+        assert abs(s.objective_value - eobj) <= 1e-6 * max(1, abs(eobj))
+        assert np.allclose(s.primal, ex, atol=1e-5)
+        assert abs(s.objective_value - s.dual_objective_value) <= 1e-5 * max(1, abs(eobj))
+    _, _, iters, restarts = LADDER[name]
+    if name != "infeasible":  # certificate timing depends on summation order
+        assert s.iterations == iters and s.restarts == restarts
+
+
+def test_solution_satisfies_constraints(hip64):
+    # PdhgSuite.scala:49-67
+    for name in OPTIMAL:
+        p, *_ = fixture(name)
+        s = hip64.solve(p, P9)
+        kx = p.to_scipy() @ s.primal
+        assert np.all(np.abs(kx[: p.n_eq] - p.q[: p.n_eq]) <= 1e-6)
+        assert np.all(kx[p.n_eq:] >= p.q[p.n_eq:] - 1e-6)
+        assert np.all(s.primal >= p.l - 1e-9) and np.all(s.primal <= p.u + 1e-9)
+
+
+def test_congestion_prices(hip64):
+    # PdhgSuite.scala:69-90
+    p, *_ = fixture("economic-dispatch")
+    s = hip64.solve(p, P9)
+    a1, b1, a2, b2 = s.dual[:4]
+    assert abs(a1 - b1) <= 1e-4 and abs(a2 - b2) > 1.0
+    assert abs(a2 - 10.0) <= 1e-3 and abs(b2 - 20.0) <= 1e-3
+
+
+def test_scaling_does_not_change_the_answer(hip64):
+    unscaled = prima.PdhgParams(eps_abs=1e-9, eps_rel=1e-9, ruiz_iterations=0,
+                                apply_pock_chambolle=False)
+    for name in OPTIMAL:
+        p, *_ = fixture(name)
+        a, b = hip64.solve(p, P9), hip64.solve(p, unscaled)
+        assert b.status == "Optimal"
+        assert abs(a.objective_value - b.objective_value) <= 1e-5 * max(1, abs(a.objective_value))
+
+
+def test_iteration_limit_exact(hip64):
+    p, *_ = fixture("economic-dispatch")
+    s = hip64.solve(p, prima.PdhgParams(eps_abs=1e-14, eps_rel=1e-14, max_iterations=3))
+    assert s.status == "IterationLimit" and s.iterations == 3
+
+
+def test_trivial_problem_zero_iterations(hip64):
+    b = prima.builder(2)
+    b.bounds(0, 0.0, 1.0).bounds(1, 0.0, 1.0)
+    s = hip64.solve(b.build()[0], P9)
+    assert s.status == "Optimal" and s.iterations == 0
+
+
+def test_time_limit(hip64):
+    # PdhgSuite.scala:124-144
+    p = prima.LpProblem.from_dict(OracleLp.random_feasible(5, 200, 40, 80, 0.3).export())
+    s = hip64.solve(p, prima.PdhgParams(eps_abs=1e-16, eps_rel=1e-16, max_iterations=50_000_000,
+                                        time_limit_ms=50))
+    assert s.status == "TimeLimit" and s.iterations < 50_000_000
+
+
+def test_invalid_arguments_raise(hip64):
+    p, *_ = fixture("economic-dispatch")
+    with pytest.raises(ValueError):
+        hip64.solve(p, prima.PdhgParams(restart_artificial=0.0))
+    with pytest.raises(ValueError):
+        hip64.solve(p, prima.PdhgParams(infeasibility_tolerance=-1e-9))
+    bad = prima.LpProblem(p.n, p.m, p.n_eq, p.row_ptr, p.col, p.val, p.c, p.q, p.u, p.l)
+    with pytest.raises(ValueError):
+        hip64.solve(bad)
+    with pytest.raises(ValueError):
+        hip64.solve(p, P9, prima.WarmStart(np.full(p.n, np.nan), np.zeros(p.m)))
+
+
+def test_spectral_bound_column(hip64):
+    # PdhgSuite.scala:155-174
+    b = prima.builder(1)
+    b.objectiveCoefficient(0, 1.0).bounds(0, 0.0, 10.0)
+    for _ in range(100):
+        b.greaterThan([(0, 1.0)], 3.0)
+    s = hip64.solve(b.build()[0], P9)
+    assert s.status == "Optimal" and abs(s.objective_value - 3.0) <= 1e-6
+
+
+def test_random_ladder_against_oracle_and_published(hip64):
+    for name, args in RANDOM_LADDER.items():
+        _, oj, _, _ = LADDER[name]
+        p = prima.LpProblem.from_dict(OracleLp.random_feasible(*args).export())
+        s = hip64.solve(p, prima.PdhgParams(eps_abs=1e-9, eps_rel=1e-9, max_iterations=500_000))
+        assert s.status == "Optimal"
+        assert gap(s.objective_value, oj) <= 1e-8, (name, s.objective_value)
+        assert s.restarts > 0
+
+
+def test_config0_closed_form_fp64(hip64):
+    for seed in range(3):
+        p = prima.config0(seed)
+        s = hip64.solve(p)
+        assert s.status == "Optimal"
+        assert gap(s.objective_value, p.known_objective) <= 1e-8
+
+
+def test_dispatch_against_highs_fp64(hip64):
+    for d in gold(lambda a: a.get("buses") == 20)[:4]:
+        p = prima.dispatch(**d["args"])
+        s = hip64.solve(p, prima.PdhgParams(max_iterations=500_000))
+        assert s.status == "Optimal"
+        assert gap(s.objective_value, d["objective"]) <= 1e-8
+
+
+def test_mixed_precision_f32_device(hip32):
+    # MixedPrecisionSuite.scala:32-60: known optima, full fp64 KKT after refinement
+    for name in OPTIMAL:
+        p, eobj, _, _ = fixture(name)
+        s = hip32.mixed_solve(p, P9)
+        assert s.status == "Optimal" and s.mixed["refined"]
+        assert abs(s.objective_value - eobj) <= 1e-6 * max(1, abs(eobj))
+    p = prima.config0(1)
+    s = hip32.mixed_solve(p, prima.PdhgParams())
+    assert s.status == "Optimal" and gap(s.objective_value, p.known_objective) <= 1e-8
+
+
+def test_mixed_precision_fp64_device_skips_reduced_pass(hip64):
+    p, *_ = fixture("economic-dispatch")
+    s = hip64.mixed_solve(p, P9)
+    assert s.status == "Optimal" and s.mixed["reduced_iterations"] == 0 and not s.mixed["refined"]
+
+
+def test_mixed_certificates_rechecked(hip32):
+    # MixedPrecisionSuite.scala:144-172: device certificates hold in fp64
+    for name, status in (("infeasible", "PrimalInfeasible"), ("unbounded", "DualInfeasible")):
+        p, *_ = fixture(name)
+        s = hip32.mixed_solve(p, P9)
+        assert s.status == status
+    p, *_ = fixture("infeasible")
+    s = hip32.mixed_solve(p, P9, prima.MixedParams(always_refine=True))
+    assert s.status == "PrimalInfeasible" and s.mixed["refined"]
+
+
+def test_float32_pass_reaches_reduced_tolerance(hip32):
+    p = prima.LpProblem.from_dict(OracleLp.random_feasible(*RANDOM_LADDER["random-600x400"]).export())
+    s = hip32.solve(p, prima.PdhgParams(eps_abs=1e-5, eps_rel=1e-5, max_iterations=100_000))
+    assert s.status == "Optimal"
+    ref = oracle_of(p).solve(default_params(eps_abs=1e-9, eps_rel=1e-9, max_iterations=500_000))
+    assert gap(s.objective_value, ref.objective) <= 1e-4
+
+
+def test_batch_keeps_order_and_matches_single(hip64):
+    probs = [prima.config0(s) for s in range(4)] + [fixture("infeasible")[0],
+                                                   fixture("economic-dispatch")[0]]
+    sols = hip64.solve_all(probs, prima.PdhgParams())
+    for p, s in zip(probs[:4], sols[:4]):
+        assert s.status == "Optimal" and gap(s.objective_value, p.known_objective) <= 1e-8
+        single = hip64.solve(p)
+        assert s.iterations == single.iterations  # same device code, same order
+        assert s.objective_value == single.objective_value
+    assert sols[4].status == "PrimalInfeasible"
+    assert sols[5].status == "Optimal" and abs(sols[5].objective_value - 3700) <= 1e-4
+
+
+def test_warm_start_from_solution(hip64):
+    p = prima.config0(2)
+    cold = hip64.solve(p, prima.PdhgParams(eps_abs=1e-6, eps_rel=1e-6))
+    warm = hip64.solve(p, prima.PdhgParams(), prima.WarmStart.of(cold))
+    assert warm.status == "Optimal" and gap(warm.objective_value, p.known_objective) <= 1e-8
+
+
+def test_deterministic_across_runs(hip64):
+    p = prima.config0(0)
+    a, b = hip64.solve(p), hip64.solve(p)
+    assert a.iterations == b.iterations and a.objective_value == b.objective_value
+    assert np.array_equal(a.primal, b.primal) and np.array_equal(a.dual, b.dual)
+
+
+@pytest.mark.parametrize("wg", [1, 2, 5, 16])
+def test_multi_workgroup_groups_agree(hip64, wg):
+    p = prima.dispatch(seed=7, buses=60, chords=30, snapshots=24, storage=15)
+    d = gold(lambda a: a.get("buses") == 60)[0]
+    s = hip64.solve(p, prima.PdhgParams(max_iterations=500_000, workgroups_per_lp=wg))
+    assert s.status == "Optimal" and s.workgroups == wg
+    assert gap(s.objective_value, d["objective"]) <= 1e-8
