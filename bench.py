@@ -49,6 +49,10 @@ def golden_objectives() -> dict:
     return out
 
 
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
 def rel_gap(a, b):
     return abs(a - b) / max(1.0, abs(b))
 
@@ -70,7 +74,6 @@ def check(sols, idx, gold, what):
 def cpu_baseline(problem: prima.LpProblem, gold: float | None):
     """The oracle (C restatement of Prima's FP64 CPU path, single thread) on
     one LP of the batch: LPs/sec on this host's cores."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle.oracle import OracleLp, default_params
     o = OracleLp.from_csr(problem.n, problem.m, problem.n_eq, problem.row_ptr, problem.col,
                           problem.val, problem.c, problem.q, problem.l, problem.u,
@@ -109,7 +112,9 @@ def main():
     k64 = prima.HipKernels(0, 64)
 
     for _ in range(args.warmup):
+        t = time.perf_counter()
         check(k32.solve_all(batch, params, mixed=mixed), range(B), gold, "warmup")
+        log(f"warmup step: {time.perf_counter() - t:.2f} s")
 
     k32.reset_stats()
     t0 = time.perf_counter()
@@ -119,6 +124,7 @@ def main():
         sols = k32.solve_all(batch, params, mixed=mixed)
         worst = max(worst, check(sols, range(B), gold, "mixed batched"))
         iters += [(s.mixed["reduced_iterations"], s.mixed["refinement_iterations"]) for s in sols]
+        log(f"step done: {time.perf_counter() - t0:.2f} s cumulative, worst gap {worst:.2e}")
     elapsed = time.perf_counter() - t0
     st = k32.stats()
     value = B * args.steps / elapsed
@@ -131,6 +137,8 @@ def main():
     check([s64], [0], gold, "fp64 unbatched")
     fp64_lps = 1.0 / t64
     st64 = k64.stats()
+    log(f"mixed batched {value:.3f} LPs/s; fp64 unbatched {fp64_lps:.3f} LPs/s "
+        f"({s64.iterations} iterations, {t64:.2f} s); kernel stats {st}")
 
     achieved = st["algorithmic_bytes"] / (st["kernel_ms"] * 1e-3) / 1e9 if st["kernel_ms"] else 0
     launches = max(1, st["launches"])
@@ -179,6 +187,7 @@ def main():
         },
     }
     if not args.no_cpu:
+        log("timing the CPU baseline (oracle, one LP, one core)")
         line["cpu_baseline"] = cpu_baseline(batch[0], gold.get(0))
     print(json.dumps(line), flush=True)
 

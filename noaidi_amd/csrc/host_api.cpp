@@ -124,7 +124,8 @@ struct HostLp {
   std::vector<double> tval;  // scaled transpose
   std::vector<double> rs, cs, sc, sq, sl, su;
   double spec_scaled = 0, spec_orig = 0, rhs_norm = 0, obj_norm = 0;
-  std::vector<int32_t> a_rb, t_rb;
+  std::vector<int32_t> a_rb, t_rb;    // block start rows (blocks + 1)
+  std::vector<int32_t> a_rbd, t_rbd;  // device descriptors
   int G = 1;
 };
 
@@ -321,6 +322,16 @@ std::vector<int32_t> partition(const std::vector<int32_t>& rb, const int32_t* pt
   return w;
 }
 
+// {first row, first nnz} pairs, one per block plus a sentinel.
+std::vector<int32_t> block_descriptors(const std::vector<int32_t>& rb, const int32_t* ptr) {
+  std::vector<int32_t> d(rb.size() * 2);
+  for (size_t b = 0; b < rb.size(); ++b) {
+    d[2 * b] = rb[b];
+    d[2 * b + 1] = ptr[rb[b]];
+  }
+  return d;
+}
+
 void prepare(const prima_lp* lp, const prima_params* p, HostLp& h) {
   h.src = lp;
   h.n = lp->n;
@@ -345,6 +356,8 @@ void prepare(const prima_lp* lp, const prima_params* p, HostLp& h) {
   h.obj_norm = norm2(lp->c, lp->n);
   h.a_rb = row_blocks(lp->m, ptr);
   h.t_rb = row_blocks(lp->n, h.tptr.data());
+  h.a_rbd = block_descriptors(h.a_rb, ptr);
+  h.t_rbd = block_descriptors(h.t_rb, h.tptr.data());
 }
 
 // Certificates.classify (Certificates.scala:207-219) in double precision on
@@ -452,8 +465,8 @@ LpLayout carve(Carver& cv, const HostLp& h, int prec) {
   L.a_col = cv.take(z * 4);
   L.t_ptr = cv.take((n + 1) * 4);
   L.t_col = cv.take(z * 4);
-  L.a_rb = cv.take(h.a_rb.size() * 4);
-  L.t_rb = cv.take(h.t_rb.size() * 4);
+  L.a_rb = cv.take(h.a_rbd.size() * 4);
+  L.t_rb = cv.take(h.t_rbd.size() * 4);
   L.a_wrb = cv.take((G + 1) * 4);
   L.t_wrb = cv.take((G + 1) * 4);
   L.a_val = cv.take(z * s);
@@ -500,12 +513,24 @@ struct SolveJob {
   double prep_ms = 0;
 };
 
-int choose_groups(prima_ctx* ctx, const prima_params* p, int64_t nnz, int capacity) {
-  if (p->workgroups_per_lp > 0) return std::min(p->workgroups_per_lp, capacity);
-  // enough rows per workgroup to hide memory latency, few enough workgroups
-  // that barrier latency does not dominate small LPs
-  int g = (int)((nnz + 24575) / 24576);
-  return std::max(1, std::min(g, capacity));
+// Workgroups per LP: about one LDS tile of nonzeros per workgroup and SpMV,
+// shrunk proportionally when the batch would not fit the device at once.
+void choose_groups(const prima_params* p, std::vector<SolveJob>& jobs, int capacity) {
+  if (p->workgroups_per_lp > 0) {
+    for (auto& J : jobs) J.h.G = std::min(p->workgroups_per_lp, capacity);
+    return;
+  }
+  const int64_t per_wg = tile_nnz();
+  int64_t total = 0;
+  for (auto& J : jobs) {
+    const int64_t g = (std::max(J.h.nnz, (int64_t)1) + per_wg - 1) / per_wg;
+    J.h.G = (int)std::max<int64_t>(1, std::min<int64_t>(g, capacity));
+    total += J.h.G;
+  }
+  if (total > capacity) {
+    const double f = (double)capacity / (double)total;
+    for (auto& J : jobs) J.h.G = std::max(1, (int)std::floor(J.h.G * f));
+  }
 }
 
 // Upload one wave, run it to completion in chunks, download the results.
@@ -570,8 +595,8 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
     HIP_TRY(ctx, put(L.a_col, lp->col, (size_t)h.nnz * 4));
     HIP_TRY(ctx, put(L.t_ptr, h.tptr.data(), ((size_t)h.n + 1) * 4));
     HIP_TRY(ctx, put(L.t_col, h.tcol.data(), (size_t)h.nnz * 4));
-    HIP_TRY(ctx, put(L.a_rb, h.a_rb.data(), h.a_rb.size() * 4));
-    HIP_TRY(ctx, put(L.t_rb, h.t_rb.data(), h.t_rb.size() * 4));
+    HIP_TRY(ctx, put(L.a_rb, h.a_rbd.data(), h.a_rbd.size() * 4));
+    HIP_TRY(ctx, put(L.t_rb, h.t_rbd.data(), h.t_rbd.size() * 4));
     HIP_TRY(ctx, put_t(L.a_val, h.aval.data(), (size_t)h.nnz));
     HIP_TRY(ctx, put_t(L.t_val, h.tval.data(), (size_t)h.nnz));
     HIP_TRY(ctx, put_t(L.c, h.sc.data(), h.n));
@@ -805,7 +830,7 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
 int run_jobs(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const prima_params* p) {
   const int occ = ctx->occ[prec == 32 ? 0 : 1];
   const int capacity = std::max(1, ctx->cu_count * occ);
-  for (auto& J : jobs) J.h.G = choose_groups(ctx, p, J.h.nnz, capacity);
+  choose_groups(p, jobs, capacity);
   std::vector<int> wave;
   int used = 0;
   for (size_t j = 0; j < jobs.size(); ++j) {

@@ -154,11 +154,15 @@ __device__ inline bool group_barrier(uint32_t* bar, int G) {
 template <typename T, typename Epi>
 __device__ inline void spmv_blocks(const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
                                    const T* __restrict__ val, const T* __restrict__ xv,
-                                   const int32_t* __restrict__ rb, int b0, int b1, T* tile,
+                                   const int32_t* __restrict__ rbd, int b0, int b1, T* tile,
                                    double* red, Epi& epi) {
+  // rbd holds {first row, first nnz} per block, so a block's extent is two
+  // independent loads rather than a row-block -> row-pointer chain.
+  const int2* __restrict__ bd = reinterpret_cast<const int2*>(rbd);
   for (int b = b0; b < b1; ++b) {
-    const int r0 = rb[b], r1 = rb[b + 1];
-    const int p0 = ptr[r0], p1 = ptr[r1];
+    const int2 d0 = bd[b], d1 = bd[b + 1];
+    const int r0 = d0.x, r1 = d1.x;
+    const int p0 = d0.y, p1 = d1.y;
     const int nnz = p1 - p0, nrows = r1 - r0;
     if (nnz > TILE) {
       T acc = T(0);
@@ -175,6 +179,21 @@ __device__ inline void spmv_blocks(const int32_t* __restrict__ ptr, const int32_
       }
       __syncthreads();
     } else {
+      int tpr = 1;
+      if (nrows < NT) {
+        tpr = NT / (nrows > 0 ? nrows : 1);
+        tpr = 1 << (31 - __clz(tpr));
+        if (tpr > 64) tpr = 64;
+      }
+      const int sub = threadIdx.x / tpr, li = threadIdx.x & (tpr - 1);
+      const int stride = NT / tpr;
+      // the first row's bounds load while the products stream in
+      const int rf = r0 + sub;
+      int sf = 0, ef = 0;
+      if (rf < r1) {
+        sf = ptr[rf] - p0;
+        ef = ptr[rf + 1] - p0;
+      }
       int k = threadIdx.x;
       for (; k + 3 * NT < nnz; k += 4 * NT) {
         const int c0 = col[p0 + k], c1 = col[p0 + k + NT], c2 = col[p0 + k + 2 * NT],
@@ -189,16 +208,12 @@ __device__ inline void spmv_blocks(const int32_t* __restrict__ ptr, const int32_
       }
       for (; k < nnz; k += NT) tile[k] = val[p0 + k] * xv[col[p0 + k]];
       __syncthreads();
-      int tpr = 1;
-      if (nrows < NT) {
-        tpr = NT / (nrows > 0 ? nrows : 1);
-        tpr = 1 << (31 - __clz(tpr));
-        if (tpr > 64) tpr = 64;
-      }
-      const int sub = threadIdx.x / tpr, li = threadIdx.x & (tpr - 1);
-      const int stride = NT / tpr;
-      for (int r = r0 + sub; r < r1; r += stride) {
-        const int s = ptr[r] - p0, e = ptr[r + 1] - p0;
+      for (int r = rf; r < r1; r += stride) {
+        int s = sf, e = ef;
+        if (r != rf) {
+          s = ptr[r] - p0;
+          e = ptr[r + 1] - p0;
+        }
         T acc = T(0);
         for (int q = s + li; q < e; q += tpr) acc += tile[q];
         for (int off = tpr >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
@@ -276,7 +291,10 @@ struct EpiDual {  // rows of A: kxBar and the projected dual step (ScalaKernels.
   __device__ void operator()(int r, T s) {
     const T yv = L.y()[r];
     T ynew = yv + sigma * (L.q()[r] - s);
-    if (r >= n_eq && !(ynew > T(0))) ynew = T(0);
+    if (r >= n_eq) {
+      const T lo = L.v.ylo ? ((const T*)L.v.ylo)[r] : T(0);
+      if (!(ynew > lo)) ynew = lo;
+    }
     const T kdx = T(0.5) * s - T(0.5) * L.kx()[r];
     const T d = ynew - yv;
     L.yn()[r] = ynew;
@@ -304,6 +322,13 @@ __device__ inline void m_eval(const Lp<T>& L, int r, T kxv, bool avg, T inv, dou
     if (sf > 0.0) e[E_PSQ_C] += sf * sf;
     if (kxo < 0.0) e[E_ROW_C] += kxo * kxo;
     if (yo < 0.0) e[E_CONE_C] += yo * yo;
+    if (v.oylo) {
+      // shifted cone y >= ylo: the correction problem's objectives carry
+      // -ylo' (K x) on the primal side and -ylo' q on the dual side
+      const double sh = -v.oylo[r];
+      e[E_CX_C] += sh * kxo;
+      e[E_QY_C] += sh * qv;
+    }
   }
   e[E_QY_C] += qv * yo;
   e[E_YN_C] += yo * yo;
@@ -320,6 +345,11 @@ __device__ inline void m_eval(const Lp<T>& L, int r, T kxv, bool avg, T inv, dou
     } else {
       const double sf = qv - kao;
       if (sf > 0.0) e[E_PSQ_A] += sf * sf;
+      if (v.oylo) {
+        const double sh = -v.oylo[r];
+        e[E_CX_A] += sh * kao;
+        e[E_QY_A] += sh * qv;
+      }
     }
     e[E_QY_A] += qv * yao;
     const double d = (double)ya - yrv;
@@ -621,7 +651,7 @@ __device__ __noinline__ int checkpoint_decide(const LpView& v, const Ranges& R, 
   *avg_out = false;
   *err_out = cur;
   if (kkt_converged(cur, v, P)) return 0;
-  const int cls = classify(E, v, P.infeas_tol);
+  const int cls = P.certificates ? classify(E, v, P.infeas_tol) : -1;
   if (cls > 0) return cls;
   const double cur_err = kkt_weighted(cur, S.weight);
   Kkt avg = cur;
@@ -676,10 +706,10 @@ __global__ __launch_bounds__(NT) void pdhg_persistent(const LpView* __restrict__
   R.tb1 = v.t_wrb[R.rank + 1];
   R.ab0 = v.a_wrb[R.rank];
   R.ab1 = v.a_wrb[R.rank + 1];
-  R.v0 = v.t_rb[R.tb0];
-  R.v1 = v.t_rb[R.tb1];
-  R.r0 = v.a_rb[R.ab0];
-  R.r1 = v.a_rb[R.ab1];
+  R.v0 = v.t_rb[2 * R.tb0];
+  R.v1 = v.t_rb[2 * R.tb1];
+  R.r0 = v.a_rb[2 * R.ab0];
+  R.r1 = v.a_rb[2 * R.ab1];
   const int G = R.G;
 
   SolveState S = *v.st;

@@ -33,7 +33,7 @@ struct KParams {
   int32_t iter_budget;   // stop after a checkpoint once this many iterations ran
   int32_t mode;          // 0: run, 1: finalize with final_status
   int32_t final_status;
-  int32_t pad;
+  int32_t certificates;  // 1: test Farkas certificates at checkpoints
 };
 
 // Device view of one LP (one group of workgroups). Pointers are device
@@ -47,10 +47,15 @@ struct LpView {
   const void* a_val;
   const int32_t *t_ptr, *t_col;
   const void* t_val;
-  const int32_t *a_rb, *t_rb;    // row-block starts (blocks + 1)
+  const int32_t *a_rb, *t_rb;    // {first row, first nnz} per row block (blocks + 1 pairs)
   const int32_t *a_wrb, *t_wrb;  // first row block of each rank (G + 1)
   const void *c, *q, *l, *u;     // scaled data, iteration precision
   const double *oc, *oq, *ol, *ou, *rs, *cs;  // original data and scales
+  // Optional lower bound on the inequality-row multipliers (default 0): the
+  // shifted dual cone of an iterative-refinement correction problem. ylo is
+  // scaled and in the iteration precision, oylo in original units.
+  const void* ylo;
+  const double* oylo;
   void *x, *y, *kx, *kty, *xn, *yn, *xbar, *kxbar, *sx, *sy, *skx, *skty, *xr, *yr;
   double *x_out, *y_out, *rc_out;
   double* part;       // 2 * G * 4 iteration partials, then G * kEvalParts
