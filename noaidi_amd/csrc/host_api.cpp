@@ -24,6 +24,7 @@ hipError_t launch_pdhg(int precision, const LpView* views, const int32_t* wg_lp,
                        const KParams& P, hipStream_t stream);
 int threads_per_block();
 int tile_nnz();
+int block_row_cap();
 // kernels_ops.hip
 hipError_t op_spmv(int prec, int rows, int64_t nnz, const int32_t* ptr, const int32_t* col,
                    const void* val, const void* x, void* out, hipStream_t s);
@@ -278,10 +279,10 @@ void equilibrate(const prima_lp* lp, const prima_params* p, HostLp& h) {
 }
 
 // Row blocks for the streamed SpMV: consecutive rows while their nnz fit the
-// LDS tile and there are at most threads_per_block of them; a row longer
-// than the tile is a block of its own.
+// LDS tile (and at most a tile's worth of rows, so empty rows cannot make a
+// block unbounded); a row longer than the tile is a block of its own.
 std::vector<int32_t> row_blocks(int32_t rows, const int32_t* ptr) {
-  const int tile = tile_nnz(), nt = threads_per_block();
+  const int tile = tile_nnz(), nt = block_row_cap();
   std::vector<int32_t> rb;
   rb.push_back(0);
   int32_t r = 0;
@@ -513,29 +514,35 @@ struct SolveJob {
   double prep_ms = 0;
 };
 
-// Workgroups per LP: about one LDS tile of nonzeros per workgroup and SpMV,
-// shrunk proportionally when the batch would not fit the device at once.
+// Workgroups per LP. Alone, an LP is latency-bound: spread it at about 2k
+// nonzeros per workgroup. In a batch that would not fit the device that way,
+// every extra workgroup of an LP costs barrier latency that other LPs could
+// have used: give each LP about 16k nonzeros per workgroup, capped by its
+// share of the device (measured on the 65k-nnz dispatch LP: 32 workgroups
+// alone, 4 in a batch of 32, 1 in a batch of 256).
 void choose_groups(const prima_params* p, std::vector<SolveJob>& jobs, int capacity) {
   if (p->workgroups_per_lp > 0) {
     for (auto& J : jobs) J.h.G = std::min(p->workgroups_per_lp, capacity);
     return;
   }
-  const int64_t per_wg = tile_nnz();
+  auto ceil_div = [](int64_t a, int64_t b) { return (a + b - 1) / b; };
   int64_t total = 0;
   for (auto& J : jobs) {
-    const int64_t g = (std::max(J.h.nnz, (int64_t)1) + per_wg - 1) / per_wg;
+    const int64_t g = ceil_div(std::max<int64_t>(J.h.nnz, 1), tile_nnz() / 2);
     J.h.G = (int)std::max<int64_t>(1, std::min<int64_t>(g, capacity));
     total += J.h.G;
   }
-  if (total > capacity) {
-    const double f = (double)capacity / (double)total;
-    for (auto& J : jobs) J.h.G = std::max(1, (int)std::floor(J.h.G * f));
+  if (total <= capacity) return;
+  const int64_t share = std::max<int64_t>(1, capacity / (int64_t)jobs.size());
+  for (auto& J : jobs) {
+    const int64_t g = ceil_div(std::max<int64_t>(J.h.nnz, 1), 4 * (int64_t)tile_nnz());
+    J.h.G = (int)std::max<int64_t>(1, std::min(g, share));
   }
 }
 
 // Upload one wave, run it to completion in chunks, download the results.
 int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::vector<int>& idx,
-             const prima_params* p) {
+             const prima_params* p, bool certificates) {
   Wave W;
   W.idx = idx;
   Carver cv;
@@ -719,6 +726,7 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
   P.max_iter = p->max_iterations;
   P.eval_freq = p->evaluation_frequency;
   P.mode = 0;
+  P.certificates = certificates ? 1 : 0;
   int budget = p->chunk_iterations > 0 ? p->chunk_iterations : 1024;
   const size_t nlp = idx.size();
   std::vector<SolveState> states(nlp);
@@ -827,7 +835,8 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
 
 // Solve a set of prepared jobs in waves that fit the device's co-resident
 // workgroup capacity.
-int run_jobs(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const prima_params* p) {
+int run_jobs(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const prima_params* p,
+             bool certificates = true) {
   const int occ = ctx->occ[prec == 32 ? 0 : 1];
   const int capacity = std::max(1, ctx->cu_count * occ);
   choose_groups(p, jobs, capacity);
@@ -835,7 +844,7 @@ int run_jobs(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const prima_
   int used = 0;
   for (size_t j = 0; j < jobs.size(); ++j) {
     if (used + jobs[j].h.G > capacity && !wave.empty()) {
-      int rc = run_wave(ctx, prec, jobs, wave, p);
+      int rc = run_wave(ctx, prec, jobs, wave, p, certificates);
       if (rc) return rc;
       wave.clear();
       used = 0;
@@ -843,7 +852,7 @@ int run_jobs(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const prima_
     wave.push_back((int)j);
     used += jobs[j].h.G;
   }
-  if (!wave.empty()) return run_wave(ctx, prec, jobs, wave, p);
+  if (!wave.empty()) return run_wave(ctx, prec, jobs, wave, p, certificates);
   return PRIMA_OK;
 }
 

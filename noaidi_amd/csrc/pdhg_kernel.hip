@@ -39,6 +39,7 @@ namespace prima {
 constexpr int NT = 512;      // threads per workgroup
 constexpr int NWAVE = NT / 64;
 constexpr int TILE = 4096;   // nnz of a streamed row block (staged in LDS)
+constexpr int ROWCAP = 2048; // rows of a streamed row block
 
 enum {
   E_PSQ_C = 0, E_QY_C, E_YN_C, E_CONE_C, E_ROW_C, E_DYR_C,
@@ -146,16 +147,19 @@ __device__ inline bool group_barrier(uint32_t* bar, int G) {
 }
 
 // ---------------------------------------------------------------------------
-// CSR SpMV over a workgroup's row blocks (CSR-Adaptive style): a block of
-// short rows streams its products into LDS with coalesced loads and each row
-// is then summed by a power-of-two sub-wave; a block holding one row longer
-// than the tile is summed by the whole workgroup. The epilogue runs once per
-// row with the row's sum.
+// CSR SpMV over a workgroup's row blocks (CSR-Adaptive style). A block of
+// short rows streams val*x[col] into LDS with coalesced, 8-deep loads while
+// its row pointers are staged in LDS; each row is then summed from LDS by a
+// power-of-two sub-wave into an LDS row-sum array; finally a coalesced pass
+// runs the epilogue with the operand loads of four rows in flight per thread.
+// A block holding one row longer than the tile is summed by the whole
+// workgroup. Epilogues are two-phase: load(r) fetches a row's operands,
+// apply(r, sum, ops) consumes them.
 template <typename T, typename Epi>
 __device__ inline void spmv_blocks(const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
                                    const T* __restrict__ val, const T* __restrict__ xv,
                                    const int32_t* __restrict__ rbd, int b0, int b1, T* tile,
-                                   double* red, Epi& epi) {
+                                   T* rsum, int* rptr, Epi& epi) {
   // rbd holds {first row, first nnz} per block, so a block's extent is two
   // independent loads rather than a row-block -> row-pointer chain.
   const int2* __restrict__ bd = reinterpret_cast<const int2*>(rbd);
@@ -169,58 +173,66 @@ __device__ inline void spmv_blocks(const int32_t* __restrict__ ptr, const int32_
       for (int p = p0 + (int)threadIdx.x; p < p1; p += NT) acc += val[p] * xv[col[p]];
       // fixed-order workgroup tree in the iteration precision
       for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
-      T* tred = tile;
-      if ((threadIdx.x & 63) == 0) tred[threadIdx.x >> 6] = acc;
+      if ((threadIdx.x & 63) == 0) tile[threadIdx.x >> 6] = acc;
       __syncthreads();
       if (threadIdx.x == 0) {
         T s = T(0);
-        for (int w = 0; w < NWAVE; ++w) s += tred[w];
-        epi(r0, s);
+        for (int w = 0; w < NWAVE; ++w) s += tile[w];
+        const typename Epi::Ops o = epi.load(r0);
+        epi.apply(r0, s, o);
       }
       __syncthreads();
-    } else {
-      int tpr = 1;
-      if (nrows < NT) {
-        tpr = NT / (nrows > 0 ? nrows : 1);
-        tpr = 1 << (31 - __clz(tpr));
-        if (tpr > 64) tpr = 64;
-      }
-      const int sub = threadIdx.x / tpr, li = threadIdx.x & (tpr - 1);
-      const int stride = NT / tpr;
-      // the first row's bounds load while the products stream in
-      const int rf = r0 + sub;
-      int sf = 0, ef = 0;
-      if (rf < r1) {
-        sf = ptr[rf] - p0;
-        ef = ptr[rf + 1] - p0;
-      }
-      int k = threadIdx.x;
-      for (; k + 3 * NT < nnz; k += 4 * NT) {
-        const int c0 = col[p0 + k], c1 = col[p0 + k + NT], c2 = col[p0 + k + 2 * NT],
-                  c3 = col[p0 + k + 3 * NT];
-        const T v0 = val[p0 + k], v1 = val[p0 + k + NT], v2 = val[p0 + k + 2 * NT],
-                v3 = val[p0 + k + 3 * NT];
-        const T x0 = xv[c0], x1 = xv[c1], x2 = xv[c2], x3 = xv[c3];
-        tile[k] = v0 * x0;
-        tile[k + NT] = v1 * x1;
-        tile[k + 2 * NT] = v2 * x2;
-        tile[k + 3 * NT] = v3 * x3;
-      }
-      for (; k < nnz; k += NT) tile[k] = val[p0 + k] * xv[col[p0 + k]];
-      __syncthreads();
-      for (int r = rf; r < r1; r += stride) {
-        int s = sf, e = ef;
-        if (r != rf) {
-          s = ptr[r] - p0;
-          e = ptr[r + 1] - p0;
-        }
-        T acc = T(0);
-        for (int q = s + li; q < e; q += tpr) acc += tile[q];
-        for (int off = tpr >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
-        if (li == 0) epi(r, acc);
-      }
-      __syncthreads();
+      continue;
     }
+    for (int r = threadIdx.x; r <= nrows; r += NT) rptr[r] = ptr[r0 + r] - p0;
+    int k = threadIdx.x;
+    for (; k + 7 * NT < nnz; k += 8 * NT) {
+      int c[8];
+      T v[8], x[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) c[j] = col[p0 + k + j * NT];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) v[j] = val[p0 + k + j * NT];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) x[j] = xv[c[j]];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) tile[k + j * NT] = v[j] * x[j];
+    }
+    for (; k + NT < nnz; k += 2 * NT) {
+      const int c0 = col[p0 + k], c1 = col[p0 + k + NT];
+      const T v0 = val[p0 + k], v1 = val[p0 + k + NT];
+      const T x0 = xv[c0], x1 = xv[c1];
+      tile[k] = v0 * x0;
+      tile[k + NT] = v1 * x1;
+    }
+    if (k < nnz) tile[k] = val[p0 + k] * xv[col[p0 + k]];
+    __syncthreads();
+    int tpr = 1;
+    if (nrows < NT) {
+      tpr = NT / (nrows > 0 ? nrows : 1);
+      tpr = 1 << (31 - __clz(tpr));
+      if (tpr > 64) tpr = 64;
+    }
+    const int li = threadIdx.x & (tpr - 1);
+    const int stride = NT / tpr;
+    for (int r = threadIdx.x / tpr; r < nrows; r += stride) {
+      const int s = rptr[r], e = rptr[r + 1];
+      T acc = T(0);
+      for (int q = s + li; q < e; q += tpr) acc += tile[q];
+      for (int off = tpr >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+      if (li == 0) rsum[r] = acc;
+    }
+    __syncthreads();
+    for (int base = threadIdx.x; base < nrows; base += 4 * NT) {
+      typename Epi::Ops o[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (base + j * NT < nrows) o[j] = epi.load(r0 + base + j * NT);
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (base + j * NT < nrows) epi.apply(r0 + base + j * NT, rsum[base + j * NT], o[j]);
+    }
+    __syncthreads();
   }
 }
 
@@ -254,11 +266,11 @@ struct Lp {
 };
 
 // Projected primal step, extrapolation and |dx|^2 (ScalaKernels.scala:78-98,
-// Pdhg.scala:341-348); x_i and kty_i are the committed values.
+// Pdhg.scala:341-348) from the committed x_i and kty_i.
 template <typename T>
-__device__ inline void primal_update(const Lp<T>& L, int i, T xi, T ktyi, T tau, T& dx2) {
-  const T st = xi - tau * (L.c()[i] - ktyi);
-  const T li = L.l()[i], ui = L.u()[i];
+__device__ inline void primal_update(const Lp<T>& L, int i, T xi, T ktyi, T ci, T li, T ui, T tau,
+                                     T& dx2) {
+  const T st = xi - tau * (ci - ktyi);
   const T xnew = st < li ? li : (st > ui ? ui : st);
   L.xn()[i] = xnew;
   L.xbar()[i] = T(2) * xnew - xi;
@@ -272,13 +284,27 @@ struct EpiCommitPrimal {  // rows of A^T: pending column-side commit, then prima
   T w, tau;
   bool primal;
   T dx2;
-  __device__ void operator()(int i, T s) {
-    const T xi = L.xn()[i];
-    L.x()[i] = xi;
+  struct Ops {
+    T xn, sx, skty, c, l, u;
+  };
+  __device__ Ops load(int i) const {
+    Ops o;
+    o.xn = L.xn()[i];
+    o.sx = L.sx()[i];
+    o.skty = L.skty()[i];
+    if (primal) {
+      o.c = L.c()[i];
+      o.l = L.l()[i];
+      o.u = L.u()[i];
+    }
+    return o;
+  }
+  __device__ void apply(int i, T s, const Ops& o) {
+    L.x()[i] = o.xn;
     L.kty()[i] = s;
-    L.sx()[i] = w * xi + L.sx()[i];
-    L.skty()[i] = w * s + L.skty()[i];
-    if (primal) primal_update(L, i, xi, s, tau, dx2);
+    L.sx()[i] = w * o.xn + o.sx;
+    L.skty()[i] = w * s + o.skty;
+    if (primal) primal_update(L, i, o.xn, s, o.c, o.l, o.u, tau, dx2);
   }
 };
 
@@ -288,15 +314,22 @@ struct EpiDual {  // rows of A: kxBar and the projected dual step (ScalaKernels.
   T sigma;
   int n_eq;
   T dy2, dykdx;
-  __device__ void operator()(int r, T s) {
-    const T yv = L.y()[r];
-    T ynew = yv + sigma * (L.q()[r] - s);
-    if (r >= n_eq) {
-      const T lo = L.v.ylo ? ((const T*)L.v.ylo)[r] : T(0);
-      if (!(ynew > lo)) ynew = lo;
-    }
-    const T kdx = T(0.5) * s - T(0.5) * L.kx()[r];
-    const T d = ynew - yv;
+  struct Ops {
+    T y, q, kx, lo;
+  };
+  __device__ Ops load(int r) const {
+    Ops o;
+    o.y = L.y()[r];
+    o.q = L.q()[r];
+    o.kx = L.kx()[r];
+    o.lo = (r >= n_eq && L.v.ylo) ? ((const T*)L.v.ylo)[r] : T(0);
+    return o;
+  }
+  __device__ void apply(int r, T s, const Ops& o) {
+    T ynew = o.y + sigma * (o.q - s);
+    if (r >= n_eq && !(ynew > o.lo)) ynew = o.lo;
+    const T kdx = T(0.5) * s - T(0.5) * o.kx;
+    const T d = ynew - o.y;
     L.yn()[r] = ynew;
     L.kxbar()[r] = s;
     dy2 += d * d;
@@ -306,12 +339,35 @@ struct EpiDual {  // rows of A: kxBar and the projected dual step (ScalaKernels.
 
 // Row-side evaluation partials in original units (Kkt.scala:68-125,
 // Certificates.scala:60-195) for the current and, optionally, the averaged
-// iterate.
+// iterate. Operands are fetched by m_ops so that several rows' loads can be
+// in flight together.
 template <typename T>
-__device__ inline void m_eval(const Lp<T>& L, int r, T kxv, bool avg, T inv, double* e) {
+struct MOps {
+  T y, yr, sy, skx;
+  double rs, q, ylo;
+};
+
+template <typename T>
+__device__ inline MOps<T> m_ops(const Lp<T>& L, int r, bool avg) {
   const LpView& v = L.v;
-  const double rs = v.rs[r], qv = v.oq[r];
-  const T yv = L.y()[r];
+  MOps<T> o;
+  o.y = L.y()[r];
+  o.yr = L.yr()[r];
+  if (avg) {
+    o.sy = L.sy()[r];
+    o.skx = L.skx()[r];
+  }
+  o.rs = v.rs[r];
+  o.q = v.oq[r];
+  o.ylo = (v.oylo && r >= v.n_eq) ? v.oylo[r] : 0.0;
+  return o;
+}
+
+template <typename T>
+__device__ inline void m_eval(const LpView& v, int r, T kxv, const MOps<T>& o, bool avg, T inv,
+                              double* e) {
+  const double rs = o.rs, qv = o.q;
+  const T yv = o.y;
   const double yo = (double)yv * rs, kxo = (double)kxv / rs;
   if (r < v.n_eq) {
     const double res = kxo - qv;
@@ -325,19 +381,19 @@ __device__ inline void m_eval(const Lp<T>& L, int r, T kxv, bool avg, T inv, dou
     if (v.oylo) {
       // shifted cone y >= ylo: the correction problem's objectives carry
       // -ylo' (K x) on the primal side and -ylo' q on the dual side
-      const double sh = -v.oylo[r];
+      const double sh = -o.ylo;
       e[E_CX_C] += sh * kxo;
       e[E_QY_C] += sh * qv;
     }
   }
   e[E_QY_C] += qv * yo;
   e[E_YN_C] += yo * yo;
-  const double yrv = (double)L.yr()[r];
+  const double yrv = (double)o.yr;
   const double dyr = (double)yv - yrv;
   e[E_DYR_C] += dyr * dyr;
   if (avg) {
-    const T ya = inv * L.sy()[r];
-    const T ka = inv * L.skx()[r];
+    const T ya = inv * o.sy;
+    const T ka = inv * o.skx;
     const double yao = (double)ya * rs, kao = (double)ka / rs;
     if (r < v.n_eq) {
       const double res = kao - qv;
@@ -346,7 +402,7 @@ __device__ inline void m_eval(const Lp<T>& L, int r, T kxv, bool avg, T inv, dou
       const double sf = qv - kao;
       if (sf > 0.0) e[E_PSQ_A] += sf * sf;
       if (v.oylo) {
-        const double sh = -v.oylo[r];
+        const double sh = -o.ylo;
         e[E_CX_A] += sh * kao;
         e[E_QY_A] += sh * qv;
       }
@@ -399,18 +455,24 @@ struct EpiFreshKx {  // rows of A: kx = A x recomputed, then row-side evaluation
   bool avg;
   T inv;
   double* e;
-  __device__ void operator()(int r, T s) {
+  using Ops = MOps<T>;
+  __device__ Ops load(int r) const { return m_ops(L, r, avg); }
+  __device__ void apply(int r, T s, const Ops& o) {
     L.kx()[r] = s;
-    m_eval(L, r, s, avg, inv, e);
+    m_eval(L.v, r, s, o, avg, inv, e);
   }
 };
 
 template <typename T>
 struct EpiInitKty {  // rows of A^T: kty = A^T y at initialisation
   const Lp<T>& L;
-  __device__ void operator()(int i, T s) {
+  struct Ops {
+    T x;
+  };
+  __device__ Ops load(int i) const { return Ops{L.x()[i]}; }
+  __device__ void apply(int i, T s, const Ops& o) {
     L.kty()[i] = s;
-    L.xr()[i] = L.x()[i];
+    L.xr()[i] = o.x;
     L.sx()[i] = T(0);
     L.skty()[i] = T(0);
   }
@@ -489,6 +551,8 @@ struct Ranges {
 
 struct Smem {
   double tile[TILE];
+  double rsum[ROWCAP];
+  int rptr[ROWCAP + 1];
   double red[NWAVE * kEvalParts + kEvalParts];
   double E[kEvalParts];  // group totals of the last evaluation
 };
@@ -507,9 +571,11 @@ __device__ __noinline__ bool eval_pass(const LpView& v, const Ranges& R, bool fr
   if (fresh_kx) {
     EpiFreshKx<T> epi{L, avg, inv, e};
     spmv_blocks(v.a_ptr, v.a_col, L.aval(), (const T*)L.x(), v.a_rb, R.ab0, R.ab1,
-                reinterpret_cast<T*>(sm.tile), sm.red, epi);
+                reinterpret_cast<T*>(sm.tile),
+                reinterpret_cast<T*>(sm.rsum), sm.rptr, epi);
   } else {
-    for (int r = R.r0 + threadIdx.x; r < R.r1; r += NT) m_eval(L, r, L.kx()[r], avg, inv, e);
+    for (int r = R.r0 + threadIdx.x; r < R.r1; r += NT)
+      m_eval(v, r, L.kx()[r], m_ops(L, r, avg), avg, inv, e);
   }
   for (int i = R.v0 + threadIdx.x; i < R.v1; i += NT) n_eval(L, i, avg, inv, e);
   block_sum<E_COUNT>(e, sm.red);
@@ -592,7 +658,8 @@ __device__ inline T commit_columns(const LpView& v, const Ranges& R, T w, bool p
   const Lp<T> L(v);
   EpiCommitPrimal<T> epi{L, w, tau, primal, T(0)};
   spmv_blocks(v.t_ptr, v.t_col, L.tval(), (const T*)L.yn(), v.t_rb, R.tb0, R.tb1,
-              reinterpret_cast<T*>(sm.tile), sm.red, epi);
+              reinterpret_cast<T*>(sm.tile),
+                reinterpret_cast<T*>(sm.rsum), sm.rptr, epi);
   return epi.dx2;
 }
 
@@ -609,7 +676,8 @@ __device__ __noinline__ void init_arrays(const LpView& v, const Ranges& R, Smem&
   const Lp<T> L(v);
   EpiInitKty<T> epi{L};
   spmv_blocks(v.t_ptr, v.t_col, L.tval(), (const T*)L.y(), v.t_rb, R.tb0, R.tb1,
-              reinterpret_cast<T*>(sm.tile), sm.red, epi);
+              reinterpret_cast<T*>(sm.tile),
+                reinterpret_cast<T*>(sm.rsum), sm.rptr, epi);
   for (int r = R.r0 + threadIdx.x; r < R.r1; r += NT) {
     L.yr()[r] = L.y()[r];
     L.sy()[r] = T(0);
@@ -749,7 +817,7 @@ __global__ __launch_bounds__(NT) void pdhg_persistent(const LpView* __restrict__
       pending = false;
     } else {
       for (int i = R.v0 + threadIdx.x; i < R.v1; i += NT)
-        primal_update(L, i, L.x()[i], L.kty()[i], tau, dx2);
+        primal_update(L, i, L.x()[i], L.kty()[i], L.c()[i], L.l()[i], L.u()[i], tau, dx2);
     }
     {
       double a[1] = {(double)dx2};
@@ -762,7 +830,7 @@ __global__ __launch_bounds__(NT) void pdhg_persistent(const LpView* __restrict__
     {
       EpiDual<T> epi{L, sigma, v.n_eq, T(0), T(0)};
       spmv_blocks(v.a_ptr, v.a_col, L.aval(), (const T*)L.xbar(), v.a_rb, R.ab0, R.ab1, tile,
-                  sm.red, epi);
+                  reinterpret_cast<T*>(sm.rsum), sm.rptr, epi);
       double a[2] = {(double)epi.dy2, (double)epi.dykdx};
       block_sum<2>(a, sm.red);
       if (threadIdx.x == 0) {
@@ -874,6 +942,7 @@ hipError_t launch_pdhg(int precision, const LpView* views, const int32_t* wg_lp,
 }
 
 int threads_per_block() { return NT; }
+int block_row_cap() { return ROWCAP; }
 int tile_nnz() { return TILE; }
 
 }  // namespace prima
