@@ -187,15 +187,28 @@ typedef struct {
   int32_t workgroups;    /* workgroups that owned this LP */
 } prima_result;
 
-/* MixedPrecision.Params (MixedPrecision.scala:37-76). */
+/* MixedPrecision.Params (MixedPrecision.scala:37-76), plus the choice of
+ * finishing pass. refinement = 0 is the reference contract: a float64 PDHG
+ * pass warm-started from the float32 result. refinement = 1 finishes with an
+ * FP64 outer iterative refinement instead (no counterpart in the reference;
+ * Gleixner, Steffy & Wolter, INFORMS J. Comput. 28(3), 2016): residuals
+ * b - Ax and c - A'y and the KKT test in float64, correction LPs on the same
+ * device matrix solved by float32 PDHG, at most max_refinements rounds, then
+ * the reference's float64 pass if the test still fails. */
 typedef struct {
   double reduced_tolerance;
   int32_t reduced_max_iterations;
   int32_t always_refine;
   double reduced_time_share;
+  int32_t refinement;
+  int32_t max_refinements;
+  double inner_tolerance;
+  int32_t inner_max_iterations;
+  int32_t pad;
 } prima_mixed_params;
 
-/* MixedPrecision.Result scalars (MixedPrecision.scala:82-91). */
+/* MixedPrecision.Result scalars (MixedPrecision.scala:82-91); rounds and
+ * fallback_iterations describe the iterative refinement (0 otherwise). */
 typedef struct {
   int32_t reduced_iterations;
   int32_t reduced_restarts;
@@ -203,6 +216,8 @@ typedef struct {
   int32_t refined;
   double reduced_ms;
   double refinement_ms;
+  int32_t rounds;
+  int32_t fallback_iterations;
 } prima_mixed_info;
 
 void prima_default_params(prima_params* p);

@@ -127,7 +127,8 @@ struct HostLp {
   double spec_scaled = 0, spec_orig = 0, rhs_norm = 0, obj_norm = 0;
   std::vector<int32_t> a_rb, t_rb;    // block start rows (blocks + 1)
   std::vector<int32_t> a_rbd, t_rbd;  // device descriptors
-  int G = 1;
+  int G = 1;      // workgroups owning this LP in the current launch
+  int G_cap = 1;  // most workgroups it may ever get (sizes its arrays)
 };
 
 // LpProblem.apply requirements (LpProblem.scala:62-104) and canonical CSR
@@ -445,6 +446,7 @@ struct LpLayout {
   size_t a_val, t_val, c, q, l, u;
   size_t x, y, kx, kty, xn, yn, xbar, kxbar, sx, sy, skx, skty, xr, yr;
   size_t oc, oq, ol, ou, rs, cs, x_out, y_out, rc_out, part, bar, st;
+  size_t ylo, oylo;
 };
 
 struct Wave {
@@ -458,9 +460,9 @@ struct Wave {
   int grid = 0;
 };
 
-LpLayout carve(Carver& cv, const HostLp& h, int prec) {
+LpLayout carve(Carver& cv, const HostLp& h, int prec, bool with_ylo) {
   const size_t s = esize(prec), n = (size_t)h.n, m = (size_t)h.m, z = (size_t)h.nnz;
-  const size_t G = (size_t)h.G;
+  const size_t G = (size_t)h.G_cap;
   LpLayout L;
   L.a_ptr = cv.take((m + 1) * 4);
   L.a_col = cv.take(z * 4);
@@ -502,6 +504,8 @@ LpLayout carve(Carver& cv, const HostLp& h, int prec) {
   L.part = cv.take((2 * G * kIterParts + G * kEvalParts) * 8);
   L.bar = cv.take(16);
   L.st = cv.take(sizeof(SolveState));
+  L.ylo = with_ylo ? cv.take(m * s) : 0;
+  L.oylo = with_ylo ? cv.take(m * 8) : 0;
   return L;
 }
 
@@ -509,6 +513,7 @@ struct SolveJob {
   const prima_lp* lp = nullptr;
   HostLp h;
   const prima_warm* warm = nullptr;
+  const double* ylo = nullptr;  // shifted dual cone for rows >= n_eq (original units)
   prima_result* res = nullptr;
   double *x_out = nullptr, *y_out = nullptr, *rc_out = nullptr;
   double prep_ms = 0;
@@ -522,7 +527,10 @@ struct SolveJob {
 // alone, 4 in a batch of 32, 1 in a batch of 256).
 void choose_groups(const prima_params* p, std::vector<SolveJob>& jobs, int capacity) {
   if (p->workgroups_per_lp > 0) {
-    for (auto& J : jobs) J.h.G = std::min(p->workgroups_per_lp, capacity);
+    for (auto& J : jobs) {
+      J.h.G = std::min(p->workgroups_per_lp, capacity);
+      J.h.G_cap = std::max(J.h.G_cap, J.h.G);
+    }
     return;
   }
   auto ceil_div = [](int64_t a, int64_t b) { return (a + b - 1) / b; };
@@ -530,6 +538,7 @@ void choose_groups(const prima_params* p, std::vector<SolveJob>& jobs, int capac
   for (auto& J : jobs) {
     const int64_t g = ceil_div(std::max<int64_t>(J.h.nnz, 1), tile_nnz() / 2);
     J.h.G = (int)std::max<int64_t>(1, std::min<int64_t>(g, capacity));
+    J.h.G_cap = std::max(J.h.G_cap, J.h.G);
     total += J.h.G;
   }
   if (total <= capacity) return;
@@ -542,11 +551,11 @@ void choose_groups(const prima_params* p, std::vector<SolveJob>& jobs, int capac
 
 // Upload one wave, run it to completion in chunks, download the results.
 int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::vector<int>& idx,
-             const prima_params* p, bool certificates) {
+             const prima_params* p, bool certificates, int capacity) {
   Wave W;
   W.idx = idx;
   Carver cv;
-  for (int j : idx) W.lay.push_back(carve(cv, jobs[j].h, prec));
+  for (int j : idx) W.lay.push_back(carve(cv, jobs[j].h, prec, jobs[j].ylo != nullptr));
   const size_t arena_bytes = cv.off;
   int grid = 0;
   for (int j : idx) grid += jobs[j].h.G;
@@ -616,6 +625,15 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
     HIP_TRY(ctx, put(L.cs, h.cs.data(), (size_t)h.n * 8));
     HIP_TRY(ctx, put(L.oq, lp->q, (size_t)h.m * 8));
     HIP_TRY(ctx, put(L.rs, h.rs.data(), (size_t)h.m * 8));
+    if (J.ylo) {
+      // y >= ylo in original units is y~ >= ylo / rowScale in scaled units
+      std::vector<double> ys((size_t)h.m, 0.0);
+      for (int32_t r = h.n_eq; r < h.m; ++r) ys[r] = J.ylo[r] / h.rs[r];
+      HIP_TRY(ctx, put_t(L.ylo, ys.data(), h.m));
+      std::vector<double> yo((size_t)h.m, 0.0);
+      for (int32_t r = h.n_eq; r < h.m; ++r) yo[r] = J.ylo[r];
+      HIP_TRY(ctx, put(L.oylo, yo.data(), (size_t)h.m * 8));
+    }
 
     // Starting point (Pdhg.scala:265-303): x0 = proj(warm / colScale or 0),
     // y0 = warm / rowScale with inequality multipliers clamped at 0.
@@ -697,6 +715,8 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
     v.part = (double*)(a + L.part);
     v.bar = (uint32_t*)(a + L.bar);
     v.st = (SolveState*)(a + L.st);
+    v.ylo = J.ylo ? (const void*)(a + L.ylo) : nullptr;
+    v.oylo = J.ylo ? (const double*)(a + L.oylo) : nullptr;
     v.offset = lp->offset;
     v.rhs_norm = h.rhs_norm;
     v.obj_norm = h.obj_norm;
@@ -706,7 +726,8 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
     wg_base += h.G;
   }
   HIP_TRY(ctx, hipMalloc((void**)&W.d_views, W.views.size() * sizeof(LpView)));
-  HIP_TRY(ctx, hipMalloc((void**)&W.d_wg_lp, W.wg_lp.size() * sizeof(int32_t)));
+  HIP_TRY(ctx, hipMalloc((void**)&W.d_wg_lp,
+                         std::max<size_t>(W.wg_lp.size(), (size_t)capacity) * sizeof(int32_t)));
   HIP_TRY(ctx, hipMemcpy(W.d_views, W.views.data(), W.views.size() * sizeof(LpView),
                          hipMemcpyHostToDevice));
   HIP_TRY(ctx, hipMemcpy(W.d_wg_lp, W.wg_lp.data(), W.wg_lp.size() * sizeof(int32_t),
@@ -730,7 +751,58 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
   int budget = p->chunk_iterations > 0 ? p->chunk_iterations : 1024;
   const size_t nlp = idx.size();
   std::vector<SolveState> states(nlp);
+  std::vector<double> done_ms(nlp, -1.0);
+  int running_groups = (int)nlp;
   auto t_solve = clk::now();
+  // Elastic batching: when LPs finish, the survivors are re-partitioned over
+  // the freed workgroups. Safe at a launch boundary, where every LP's state
+  // is committed; only the ownership tables and group sizes change.
+  auto regroup = [&]() -> int {
+    std::vector<SolveJob> live;
+    std::vector<size_t> which;
+    for (size_t k = 0; k < nlp; ++k)
+      if (states[k].status < 0) which.push_back(k);
+    if (which.empty()) return PRIMA_OK;
+    std::vector<int> newG(which.size());
+    {
+      // same policy as the initial choice, over the LPs still running
+      std::vector<SolveJob> tmp(which.size());
+      for (size_t j = 0; j < which.size(); ++j) {
+        tmp[j].h.nnz = jobs[idx[which[j]]].h.nnz;
+        tmp[j].h.G_cap = jobs[idx[which[j]]].h.G_cap;
+      }
+      choose_groups(p, tmp, capacity);
+      for (size_t j = 0; j < which.size(); ++j)
+        newG[j] = std::min(tmp[j].h.G, jobs[idx[which[j]]].h.G_cap);
+    }
+    bool changed = which.size() != (size_t)running_groups;
+    for (size_t j = 0; j < which.size(); ++j) changed |= newG[j] != jobs[idx[which[j]]].h.G;
+    if (!changed) return PRIMA_OK;
+    running_groups = (int)which.size();
+    W.wg_lp.clear();
+    int base = 0;
+    for (size_t j = 0; j < which.size(); ++j) {
+      const size_t k = which[j];
+      SolveJob& J = jobs[idx[k]];
+      J.h.G = newG[j];
+      static const int32_t zero_ptr[1] = {0};
+      const int32_t* aptr = J.lp->m > 0 ? J.lp->row_ptr : zero_ptr;
+      std::vector<int32_t> awrb = partition(J.h.a_rb, aptr, J.h.G);
+      std::vector<int32_t> twrb = partition(J.h.t_rb, J.h.tptr.data(), J.h.G);
+      HIP_TRY(ctx, put(W.lay[k].a_wrb, awrb.data(), awrb.size() * 4));
+      HIP_TRY(ctx, put(W.lay[k].t_wrb, twrb.data(), twrb.size() * 4));
+      W.views[k].G = J.h.G;
+      W.views[k].wg_base = base;
+      for (int g = 0; g < J.h.G; ++g) W.wg_lp.push_back((int32_t)k);
+      base += J.h.G;
+    }
+    W.grid = base;
+    HIP_TRY(ctx, hipMemcpy(W.d_views, W.views.data(), W.views.size() * sizeof(LpView),
+                           hipMemcpyHostToDevice));
+    HIP_TRY(ctx, hipMemcpy(W.d_wg_lp, W.wg_lp.data(), W.wg_lp.size() * sizeof(int32_t),
+                           hipMemcpyHostToDevice));
+    return PRIMA_OK;
+  };
   for (;;) {
     P.iter_budget = budget;
     auto t_l = clk::now();
@@ -745,9 +817,11 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
       ctx->stats.kernel_ms += ev_ms;
       ctx->stats.launches += 1;
     }
-    for (size_t k = 0; k < nlp; ++k)
+    for (size_t k = 0; k < nlp; ++k) {
       HIP_TRY(ctx, hipMemcpy(&states[k], W.arena + W.lay[k].st, sizeof(SolveState),
                              hipMemcpyDeviceToHost));
+      if (states[k].status >= 0 && done_ms[k] < 0) done_ms[k] = ms_since(t_solve);
+    }
     bool all_done = true;
     for (auto& st : states) all_done &= st.status >= 0;
     if (all_done) break;
@@ -773,6 +847,10 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
         HIP_TRY(ctx, hipMemcpy(&states[k], W.arena + W.lay[k].st, sizeof(SolveState),
                                hipMemcpyDeviceToHost));
       break;
+    }
+    {
+      const int rc = regroup();
+      if (rc) return rc;
     }
     if (p->chunk_iterations == 0) {
       // aim for launches of roughly 50 ms (shorter under a time limit)
@@ -822,8 +900,8 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
     r->dual_residual = st.kkt[1];
     r->objective = st.kkt[2];
     r->dual_objective = st.kkt[3];
-    r->solve_time_ms = solve_ms;
-    r->total_time_ms = solve_ms + upload_ms + J.prep_ms;
+    r->solve_time_ms = done_ms[k] >= 0 ? done_ms[k] : solve_ms;
+    r->total_time_ms = r->solve_time_ms + upload_ms + J.prep_ms;
     r->final_step_size = st.step;
     r->final_primal_weight = st.weight;
     r->trials = st.total_trials;
@@ -844,7 +922,7 @@ int run_jobs(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const prima_
   int used = 0;
   for (size_t j = 0; j < jobs.size(); ++j) {
     if (used + jobs[j].h.G > capacity && !wave.empty()) {
-      int rc = run_wave(ctx, prec, jobs, wave, p, certificates);
+      int rc = run_wave(ctx, prec, jobs, wave, p, certificates, capacity);
       if (rc) return rc;
       wave.clear();
       used = 0;
@@ -852,7 +930,7 @@ int run_jobs(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const prima_
     wave.push_back((int)j);
     used += jobs[j].h.G;
   }
-  if (!wave.empty()) return run_wave(ctx, prec, jobs, wave, p, certificates);
+  if (!wave.empty()) return run_wave(ctx, prec, jobs, wave, p, certificates, capacity);
   return PRIMA_OK;
 }
 
@@ -892,6 +970,11 @@ prima_params reduced_params(const prima_params* p, const prima_mixed_params* mx)
 }
 
 std::string validate_mixed(const prima_mixed_params* mx) {
+  if (mx->refinement != 0 && mx->refinement != 1) return "refinement must be 0 or 1";
+  if (mx->refinement == 1 && (mx->max_refinements < 0 || !(mx->inner_tolerance > 0.0) ||
+                              mx->inner_max_iterations <= 0))
+    return "iterative refinement needs max_refinements >= 0, inner_tolerance > 0 and "
+           "inner_max_iterations > 0";
   if (!(mx->reduced_tolerance > 0.0)) return "reducedTolerance must be positive";
   if (mx->reduced_max_iterations <= 0) return "reducedMaxIterations must be positive";
   if (!(mx->reduced_time_share > 0.0 && mx->reduced_time_share < 1.0))
@@ -988,6 +1071,304 @@ int mixed_batch(prima_ctx* ctx, int32_t count, const prima_lp* lps, const prima_
   return PRIMA_OK;
 }
 
+// ---------------------------------------------------------------------------
+// FP64 outer iterative refinement (Gleixner, Steffy & Wolter 2016) around
+// float32 device solves of correction LPs on the same matrix.
+
+struct Transposed {
+  std::vector<int32_t> ptr, col;
+  std::vector<double> val;
+};
+
+struct Kkt4 {
+  double pres, dres, pobj, dobj;
+};
+
+// Kkt.evaluate (Kkt.scala:68-125) in double precision on the original LP,
+// also returning K x and K'y.
+Kkt4 eval64(const prima_lp* lp, const Transposed& T, const double* x, const double* y,
+            std::vector<double>& ax, std::vector<double>& aty) {
+  const int32_t n = lp->n, m = lp->m;
+  ax.assign((size_t)m, 0.0);
+  aty.assign((size_t)n, 0.0);
+  for (int32_t r = 0; r < m; ++r) {
+    double acc = 0.0;
+    for (int32_t q = lp->row_ptr[r]; q < lp->row_ptr[r + 1]; ++q) acc += lp->val[q] * x[lp->col[q]];
+    ax[r] = acc;
+  }
+  for (int32_t i = 0; i < n; ++i) {
+    double acc = 0.0;
+    for (int32_t q = T.ptr[i]; q < T.ptr[i + 1]; ++q) acc += T.val[q] * y[T.col[q]];
+    aty[i] = acc;
+  }
+  double psq = 0.0;
+  for (int32_t r = 0; r < m; ++r) {
+    if (r < lp->n_eq) {
+      const double d = ax[r] - lp->q[r];
+      psq += d * d;
+    } else {
+      const double sf = lp->q[r] - ax[r];
+      if (sf > 0.0) psq += sf * sf;
+    }
+  }
+  double pobj = lp->offset, dobj = lp->offset, dsq = 0.0;
+  for (int32_t i = 0; i < n; ++i) pobj += lp->c[i] * x[i];
+  for (int32_t r = 0; r < m; ++r) dobj += lp->q[r] * y[r];
+  for (int32_t i = 0; i < n; ++i) {
+    const double rc = lp->c[i] - aty[i];
+    if (rc > 0.0) {
+      if (std::isinf(lp->l[i]) && lp->l[i] < 0) dsq += rc * rc;
+      else dobj += lp->l[i] * rc;
+    } else if (rc < 0.0) {
+      if (std::isinf(lp->u[i]) && lp->u[i] > 0) dsq += rc * rc;
+      else dobj += lp->u[i] * rc;
+    }
+  }
+  return Kkt4{std::sqrt(psq), std::sqrt(dsq), pobj, dobj};
+}
+
+bool kkt4_converged(const Kkt4& e, double rhs_norm, double obj_norm, const prima_params* p) {
+  const bool finite = std::isfinite(e.pres) && std::isfinite(e.dres) && std::isfinite(e.pobj) &&
+                      std::isfinite(e.dobj);
+  return finite && e.pres <= p->eps_abs + p->eps_rel * rhs_norm &&
+         e.dres <= p->eps_abs + p->eps_rel * obj_norm &&
+         std::fabs(e.pobj - e.dobj) <= p->eps_abs + p->eps_rel * (std::fabs(e.pobj) + std::fabs(e.dobj));
+}
+
+struct RefineState {
+  std::vector<double> x, y, ax, aty;
+  Transposed T;
+  double dp = 1.0, dd = 1.0, rhs_norm = 0.0, obj_norm = 0.0;
+  bool active = true;
+  int inner_iterations = 0, inner_restarts = 0, rounds = 0;
+  double step = NAN, weight = NAN;
+  // correction LP storage
+  std::vector<double> cc, cq, cl, cu, ylo, cx, cy, crc;
+  prima_lp corr{};
+  prima_result cres{};
+};
+
+int refine_batch(prima_ctx* ctx, int32_t count, const prima_lp* lps, const prima_params* p,
+                 const prima_mixed_params* mx, const prima_warm* warms, prima_result* results,
+                 prima_mixed_info* infos, double** xo, double** yo, double** rco) {
+  auto t0 = clk::now();
+  const prima_params rp = reduced_params(p, mx);
+  std::vector<RefineState> S(count);
+  std::vector<prima_result> red(count);
+  {
+    std::vector<SolveJob> jobs(count);
+    for (int32_t k = 0; k < count; ++k) {
+      int rc = prepare_job(ctx, jobs[k], &lps[k], &rp, warms ? &warms[k] : nullptr);
+      if (rc) return rc;
+      S[k].x.assign((size_t)lps[k].n, 0.0);
+      S[k].y.assign((size_t)lps[k].m, 0.0);
+      S[k].crc.assign((size_t)lps[k].n, 0.0);
+      jobs[k].res = &red[k];
+      jobs[k].x_out = S[k].x.data();
+      jobs[k].y_out = S[k].y.data();
+      jobs[k].rc_out = S[k].crc.data();
+    }
+    int rc = run_jobs(ctx, 32, jobs, &rp);
+    if (rc) return rc;
+  }
+  const double reduced_ms = ms_since(t0);
+  std::vector<int> fallback;
+  for (int32_t k = 0; k < count; ++k) {
+    prima_mixed_info& info = infos[k];
+    std::memset(&info, 0, sizeof(info));
+    info.reduced_iterations = red[k].iterations;
+    info.reduced_restarts = red[k].restarts;
+    info.reduced_ms = red[k].solve_time_ms;
+    info.refined = 1;
+    RefineState& R = S[k];
+    const int st = red[k].status;
+    const bool holds = conclusive(st) && st != PRIMA_OPTIMAL &&
+                       host_classify(&lps[k], R.x.data(), R.y.data(),
+                                     p->infeasibility_tolerance) == st;
+    if (!mx->always_refine && holds) {  // MixedPrecision.scala:143-165
+      results[k] = red[k];
+      info.refined = 0;
+      if (xo && xo[k]) std::memcpy(xo[k], R.x.data(), R.x.size() * 8);
+      if (yo && yo[k]) std::memcpy(yo[k], R.y.data(), R.y.size() * 8);
+      if (rco && rco[k]) std::memcpy(rco[k], R.crc.data(), R.crc.size() * 8);
+      R.active = false;
+      continue;
+    }
+    bool finite = true;
+    for (double v : R.x) finite &= std::isfinite(v);
+    for (double v : R.y) finite &= std::isfinite(v);
+    if (!finite) {  // a diverged float32 pass: the float64 pass runs cold
+      std::fill(R.x.begin(), R.x.end(), 0.0);
+      std::fill(R.y.begin(), R.y.end(), 0.0);
+      R.active = false;
+      fallback.push_back(k);
+      continue;
+    }
+    const prima_lp* lp = &lps[k];
+    static const int32_t zero_ptr[1] = {0};
+    transpose_csr(lp->m, lp->n, lp->m > 0 ? lp->row_ptr : zero_ptr, lp->col, lp->val, R.T.ptr,
+                  R.T.col, R.T.val);
+    R.rhs_norm = norm2(lp->q, lp->m);
+    R.obj_norm = norm2(lp->c, lp->n);
+    R.step = red[k].final_step_size;
+    R.weight = red[k].final_primal_weight;
+  }
+
+  prima_params ip = *p;
+  ip.eps_abs = mx->inner_tolerance;
+  ip.eps_rel = mx->inner_tolerance;
+  ip.max_iterations = mx->inner_max_iterations;
+  ip.time_limit_ms = -1;
+  const double growth = 1e4, cost_cap = 1e6;
+  for (int round = 0; round <= mx->max_refinements; ++round) {
+    std::vector<int> act;
+    for (int32_t k = 0; k < count; ++k) {
+      RefineState& R = S[k];
+      if (!R.active) continue;
+      const prima_lp* lp = &lps[k];
+      const Kkt4 e = eval64(lp, R.T, R.x.data(), R.y.data(), R.ax, R.aty);
+      if (kkt4_converged(e, R.rhs_norm, R.obj_norm, p)) {
+        prima_result& r = results[k];
+        std::memset(&r, 0, sizeof(r));
+        r.status = PRIMA_OPTIMAL;
+        r.iterations = red[k].iterations + R.inner_iterations;
+        r.restarts = red[k].restarts + R.inner_restarts;
+        r.objective = e.pobj;
+        r.dual_objective = e.dobj;
+        r.primal_residual = e.pres;
+        r.dual_residual = e.dres;
+        r.final_step_size = R.step;
+        r.final_primal_weight = R.weight;
+        r.workgroups = red[k].workgroups;
+        if (xo && xo[k]) std::memcpy(xo[k], R.x.data(), R.x.size() * 8);
+        if (yo && yo[k]) std::memcpy(yo[k], R.y.data(), R.y.size() * 8);
+        if (rco && rco[k])
+          for (int32_t i = 0; i < lp->n; ++i) rco[k][i] = lp->c[i] - R.aty[i];
+        infos[k].rounds = R.rounds;
+        infos[k].refinement_iterations = R.inner_iterations;
+        R.active = false;
+        continue;
+      }
+      if (round == mx->max_refinements) {
+        R.active = false;
+        fallback.push_back(k);
+        continue;
+      }
+      // scaled residual problem: primal and dual violations brought to O(1)
+      const int32_t n = lp->n, m = lp->m, ne = lp->n_eq;
+      double pv = 0.0, dv = 0.0;
+      for (int32_t r = 0; r < m; ++r) {
+        const double rr = lp->q[r] - R.ax[r];
+        pv = std::max(pv, r < ne ? std::fabs(rr) : std::max(rr, 0.0));
+      }
+      for (int32_t i = 0; i < n; ++i) {
+        const double rc = lp->c[i] - R.aty[i];
+        const double lo = lp->l[i], hi = lp->u[i];
+        const bool atl = std::isfinite(lo) && R.x[i] - lo <= 1e-12 * (1.0 + std::fabs(lo));
+        const bool atu = std::isfinite(hi) && hi - R.x[i] <= 1e-12 * (1.0 + std::fabs(hi));
+        double vi;
+        if (atl && atu) vi = 0.0;
+        else if (atl) vi = std::max(-rc, 0.0);
+        else if (atu) vi = std::max(rc, 0.0);
+        else vi = std::fabs(rc);
+        dv = std::max(dv, vi);
+      }
+      R.dp = std::min(1.0 / std::max(pv, 1e-300), growth * R.dp);
+      R.dd = std::min(1.0 / std::max(dv, 1e-300), growth * R.dd);
+      R.cc.resize(n);
+      R.cl.resize(n);
+      R.cu.resize(n);
+      R.cq.resize(m);
+      R.ylo.assign((size_t)m, 0.0);
+      for (int32_t i = 0; i < n; ++i) {
+        const double c = R.dd * (lp->c[i] - R.aty[i]);
+        R.cc[i] = std::max(-cost_cap, std::min(cost_cap, c));
+        R.cl[i] = R.dp * (lp->l[i] - R.x[i]);
+        R.cu[i] = R.dp * (lp->u[i] - R.x[i]);
+        if (R.cl[i] > 0.0) R.cl[i] = 0.0;  // x is inside its box
+        if (R.cu[i] < 0.0) R.cu[i] = 0.0;
+      }
+      for (int32_t r = 0; r < m; ++r) R.cq[r] = R.dp * (lp->q[r] - R.ax[r]);
+      for (int32_t r = ne; r < m; ++r) R.ylo[r] = -R.dd * R.y[r];
+      R.corr = *lp;
+      R.corr.c = R.cc.data();
+      R.corr.q = R.cq.data();
+      R.corr.l = R.cl.data();
+      R.corr.u = R.cu.data();
+      R.corr.offset = 0.0;
+      R.cx.assign((size_t)n, 0.0);
+      R.cy.assign((size_t)m, 0.0);
+      act.push_back(k);
+    }
+    if (act.empty()) break;
+    std::vector<SolveJob> jobs(act.size());
+    for (size_t j = 0; j < act.size(); ++j) {
+      RefineState& R = S[act[j]];
+      int rc = prepare_job(ctx, jobs[j], &R.corr, &ip, nullptr);
+      if (rc) return rc;
+      jobs[j].ylo = R.ylo.data();
+      jobs[j].res = &R.cres;
+      jobs[j].x_out = R.cx.data();
+      jobs[j].y_out = R.cy.data();
+    }
+    int rc = run_jobs(ctx, 32, jobs, &ip, /*certificates=*/false);
+    if (rc) return rc;
+    for (int k : act) {
+      RefineState& R = S[k];
+      const prima_lp* lp = &lps[k];
+      R.rounds++;
+      R.inner_iterations += R.cres.iterations;
+      R.inner_restarts += R.cres.restarts;
+      for (int32_t i = 0; i < lp->n; ++i) {
+        const double v = R.x[i] + R.cx[i] / R.dp;
+        R.x[i] = v < lp->l[i] ? lp->l[i] : (v > lp->u[i] ? lp->u[i] : v);
+      }
+      for (int32_t r = 0; r < lp->m; ++r) {
+        const double v = R.y[r] + R.cy[r] / R.dd;
+        R.y[r] = (r >= lp->n_eq && v < 0.0) ? 0.0 : v;
+      }
+    }
+  }
+  const double refine_ms = ms_since(t0) - reduced_ms;
+  for (int32_t k = 0; k < count; ++k) {
+    if (std::find(fallback.begin(), fallback.end(), k) != fallback.end()) continue;
+    if (infos[k].refined) {
+      results[k].solve_time_ms = ms_since(t0);
+      results[k].total_time_ms = results[k].solve_time_ms;
+      infos[k].refinement_ms = refine_ms;
+    }
+  }
+  if (fallback.empty()) return PRIMA_OK;
+  // the reference's float64 pass, warm-started from the refined point
+  std::vector<prima_warm> fw(fallback.size());
+  std::vector<SolveJob> jobs(fallback.size());
+  std::vector<prima_result> fres(fallback.size());
+  for (size_t j = 0; j < fallback.size(); ++j) {
+    const int k = fallback[j];
+    RefineState& R = S[k];
+    fw[j] = prima_warm{R.x.data(), R.y.data(), R.step, R.weight};
+    int rc = prepare_job(ctx, jobs[j], &lps[k], p, &fw[j]);
+    if (rc) return rc;
+    jobs[j].res = &fres[j];
+    jobs[j].x_out = xo ? xo[k] : nullptr;
+    jobs[j].y_out = yo ? yo[k] : nullptr;
+    jobs[j].rc_out = rco ? rco[k] : nullptr;
+  }
+  int rc = run_jobs(ctx, 64, jobs, p);
+  if (rc) return rc;
+  for (size_t j = 0; j < fallback.size(); ++j) {
+    const int k = fallback[j];
+    results[k] = fres[j];
+    infos[k].fallback_iterations = fres[j].iterations;
+    infos[k].refinement_iterations = S[k].inner_iterations + fres[j].iterations;
+    infos[k].rounds = S[k].rounds;
+    infos[k].refinement_ms = ms_since(t0) - reduced_ms;
+    results[k].solve_time_ms = ms_since(t0);
+    results[k].total_time_ms = results[k].solve_time_ms;
+  }
+  return PRIMA_OK;
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -1015,10 +1396,15 @@ void prima_default_params(prima_params* p) {
 }
 
 void prima_default_mixed_params(prima_mixed_params* p) {
+  std::memset(p, 0, sizeof(*p));
   p->reduced_tolerance = 1e-5;
   p->reduced_max_iterations = 20000;
   p->always_refine = 0;
   p->reduced_time_share = 0.5;
+  p->refinement = 0;
+  p->max_refinements = 4;
+  p->inner_tolerance = 1e-4;
+  p->inner_max_iterations = 20000;
 }
 
 int prima_open(int32_t device, int32_t precision, prima_ctx** out) {
@@ -1338,6 +1724,10 @@ int prima_solve_batch(prima_ctx* ctx, int32_t count, const prima_lp* lps,
         tmp[k].refinement_iterations = results[k].iterations;
         tmp[k].refinement_ms = results[k].solve_time_ms;
       }
+    } else if (mixed->refinement == 1) {
+      rc = refine_batch(ctx, count, lps, params, mixed, warms, results, tmp.data(), x_outs, y_outs,
+                        rc_outs);
+      if (rc) return rc;
     } else {
       rc = mixed_batch(ctx, count, lps, params, mixed, warms, results, tmp.data(), x_outs, y_outs,
                        rc_outs);

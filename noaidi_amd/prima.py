@@ -96,13 +96,17 @@ class _Result(C.Structure):
 
 class _Mixed(C.Structure):
     _fields_ = [("reduced_tolerance", C.c_double), ("reduced_max_iterations", C.c_int32),
-                ("always_refine", C.c_int32), ("reduced_time_share", C.c_double)]
+                ("always_refine", C.c_int32), ("reduced_time_share", C.c_double),
+                ("refinement", C.c_int32), ("max_refinements", C.c_int32),
+                ("inner_tolerance", C.c_double), ("inner_max_iterations", C.c_int32),
+                ("pad", C.c_int32)]
 
 
 class _MixedInfo(C.Structure):
     _fields_ = [("reduced_iterations", C.c_int32), ("reduced_restarts", C.c_int32),
                 ("refinement_iterations", C.c_int32), ("refined", C.c_int32),
-                ("reduced_ms", C.c_double), ("refinement_ms", C.c_double)]
+                ("reduced_ms", C.c_double), ("refinement_ms", C.c_double),
+                ("rounds", C.c_int32), ("fallback_iterations", C.c_int32)]
 
 
 class _Stats(C.Structure):
@@ -448,10 +452,18 @@ def builder(num_variables: int) -> LpBuilder:
 
 @dataclass
 class MixedParams:
+    """MixedPrecision.Params; `refinement` selects the finishing pass:
+    "warm" (the reference's float64 warm-started pass) or "iterative" (FP64
+    outer iterative refinement around float32 correction solves)."""
+
     reduced_tolerance: float = 1e-5
     reduced_max_iterations: int = 20_000
     always_refine: bool = False
     reduced_time_share: float = 0.5
+    refinement: str = "warm"
+    max_refinements: int = 4
+    inner_tolerance: float = 1e-4
+    inner_max_iterations: int = 20_000
 
     def _c(self) -> _Mixed:
         m = _Mixed()
@@ -459,6 +471,12 @@ class MixedParams:
         m.reduced_max_iterations = self.reduced_max_iterations
         m.always_refine = 1 if self.always_refine else 0
         m.reduced_time_share = self.reduced_time_share
+        if self.refinement not in ("warm", "iterative"):
+            raise ValueError(f"refinement must be 'warm' or 'iterative', got {self.refinement!r}")
+        m.refinement = 1 if self.refinement == "iterative" else 0
+        m.max_refinements = self.max_refinements
+        m.inner_tolerance = self.inner_tolerance
+        m.inner_max_iterations = self.inner_max_iterations
         return m
 
 
@@ -528,7 +546,7 @@ def _solution(r: _Result, x, y, rc, info: Optional[_MixedInfo] = None) -> LpSolu
                        reduced_restarts=info.reduced_restarts,
                        refinement_iterations=info.refinement_iterations,
                        refined=bool(info.refined), reduced_ms=info.reduced_ms,
-                       refinement_ms=info.refinement_ms)
+                       refinement_ms=info.refinement_ms, rounds=info.rounds)
     return s
 
 
