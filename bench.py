@@ -97,6 +97,56 @@ def cpu_baseline(problem: prima.LpProblem):
                 cpu=platform.processor() or platform.machine())
 
 
+SCHEDULES = {0: "auto", 1: "vector-1", 2: "vector-2", 3: "vector-4", 4: "vector-8",
+             5: "vector-16", 6: "vector-32", 7: "vector-64", 8: "row-blocks"}
+
+
+def spmv_roofline(reps: int = 20):
+    """The hot SpMV at the large-LP scale: A of the 2M-variable, 1M-row random
+    LP (lognormal row lengths, sigma 1, mean about 20, capped at 2048). The
+    autotuner times every schedule; the best is re-timed with HIP events.
+    Algorithmic bytes per SpMV: Z(s+4) + 4(R+1) + s*C + s*R."""
+    import math
+    p = prima.random_known(2, 2_000_000, 1_000_000, row_len=0, mu=math.log(20.0) - 0.5,
+                           sigma=1.0, max_len=2048)
+    R, Cn, Z = p.m, p.n, p.nnz
+    out = {"matrix": f"{R} x {Cn}, {Z} nnz, row lengths lognormal(ln 20 - 0.5, 1) <= 2048"}
+    rng = np.random.default_rng(0)
+    xh = rng.standard_normal(Cn)
+    for prec in (32, 64):
+        k = prima.HipKernels(0, prec)
+        A = k.uploadMatrix(R, Cn, p.row_ptr, p.col, p.val)
+        x, y = k.upload(xh), k.allocate(R)
+        times = {}
+        for sch in range(1, 9):
+            k.set_schedule(A, sch)
+            k.spmv(A, x, y)
+            k.download(y)
+            t = time.perf_counter()
+            for _ in range(reps):
+                k.spmv(A, x, y)
+            k.download(y)
+            times[SCHEDULES[sch]] = (time.perf_counter() - t) / reps * 1e3
+        choice, best_ms = k.autotune(A, reps)
+        sb = 4 if prec == 32 else 8
+        bytes_ = Z * (sb + 4) + 4 * (R + 1) + sb * Cn + sb * R
+        ref = p.to_scipy() @ xh
+        got = k.download(y) if False else None
+        k.set_schedule(A, choice)
+        k.spmv(A, x, y)
+        got = k.download(y)
+        err = float(np.max(np.abs(got - ref) / (np.abs(p.to_scipy()) @ np.abs(xh) + 1e-30)))
+        out[f"f{prec}"] = {"schedule": SCHEDULES[choice], "ms": best_ms,
+                           "achieved_GBs": bytes_ / (best_ms * 1e-3) / 1e9,
+                           "frac_of_8TBs": bytes_ / (best_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                           "algorithmic_bytes": bytes_, "max_rel_err_vs_f64_host": err,
+                           "wall_ms_per_schedule": {k_: round(v, 3) for k_, v in times.items()}}
+        log(f"spmv f{prec}: {out[f'f{prec}']}")
+        del A, x, y
+        k.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,11 +157,17 @@ def main():
     ap.add_argument("--unbatched", type=int, default=4,
                     help="scenarios solved one at a time for the FP64 denominator")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-spmv", action="store_true")
+    ap.add_argument("--only-spmv", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on exactly one GPU: this build solves independent LPs per GPU "
               "and has no multi-GPU path to measure", file=sys.stderr)
         sys.exit(2)
+
+    if args.only_spmv:
+        print(json.dumps({"spmv_roofline": spmv_roofline()}), flush=True)
+        return
 
     B = args.batch
     batch = [scenario(k) for k in range(B)]
@@ -199,6 +255,8 @@ def main():
             "unit": "GB/s",
         },
     }
+    if not args.no_spmv:
+        line["spmv_roofline"] = spmv_roofline()
     if not args.no_cpu:
         log("timing the CPU baseline (oracle, one LP, one core)")
         line["cpu_baseline"] = cpu_baseline(batch[0])

@@ -97,6 +97,21 @@ int prima_mat_upload_csr(prima_ctx* ctx, int32_t rows, int32_t cols, int64_t nnz
                          prima_mat** out);
 int prima_mat_free(prima_ctx* ctx, prima_mat* m);
 
+/* SpMV schedules for prima_spmv (no counterpart in the reference, whose
+ * SpMV is a sequential loop): sub-waves of 1..64 lanes per row, or the
+ * row-block (CSR-Adaptive) kernel the on-device solver uses. */
+#define PRIMA_SPMV_AUTO 0
+#define PRIMA_SPMV_VECTOR_1 1
+#define PRIMA_SPMV_VECTOR_64 7
+#define PRIMA_SPMV_ROW_BLOCKS 8
+/* Force a schedule for a matrix. */
+int prima_mat_set_schedule(prima_ctx* ctx, prima_mat* a, int32_t schedule);
+/* Autotuner: time every schedule on this matrix (reps launches each, HIP
+ * events on the context stream) and keep the fastest for later prima_spmv
+ * calls. Returns the choice and its time per SpMV in milliseconds. */
+int prima_mat_autotune(prima_ctx* ctx, prima_mat* a, int32_t reps, int32_t* chosen,
+                       double* best_ms);
+
 /* ---- the eight operations (kernels/Kernels.scala:73-123) ---------------- */
 
 /* Kernels.spmv: out := a * x. */

@@ -37,6 +37,11 @@ hipError_t op_dual(int prec, int64_t m, const void* y, const void* kxb, const vo
                    double sigma, int64_t n_eq, void* out, hipStream_t s);
 hipError_t op_dot(int prec, int64_t n, const void* x, const void* y, double* part,
                   hipStream_t s);
+hipError_t op_spmv_vector(int prec, int tpr, int rows, const int32_t* ptr, const int32_t* col,
+                          const void* val, const void* x, void* out, hipStream_t s);
+hipError_t op_spmv_blocks(int prec, int nblocks, int bpw, const int32_t* ptr, const int32_t* col,
+                          const void* val, const int32_t* rbd, const void* x, void* out,
+                          hipStream_t s);
 }  // namespace prima
 
 using namespace prima;
@@ -68,6 +73,9 @@ struct prima_mat {
   int32_t* ptr = nullptr;
   int32_t* col = nullptr;
   void* val = nullptr;
+  int32_t* rbd = nullptr;  // row-block descriptors for the row-block schedule
+  int32_t nblocks = 0;
+  int32_t schedule = 0;    // PRIMA_SPMV_* (0: by mean row length)
 };
 
 namespace {
@@ -1071,6 +1079,19 @@ int mixed_batch(prima_ctx* ctx, int32_t count, const prima_lp* lps, const prima_
   return PRIMA_OK;
 }
 
+// The seam SpMV in a given schedule: 0 picks a sub-wave width from the mean
+// row length, 1..7 are sub-waves of 1, 2, ..., 64 lanes per row, 8 is the
+// row-block (CSR-Adaptive) kernel the solver uses.
+hipError_t spmv_scheduled(prima_ctx* ctx, const prima_mat* a, const void* x, void* out, int sch) {
+  if (sch == 0)
+    return op_spmv(ctx->precision, a->rows, a->nnz, a->ptr, a->col, a->val, x, out, ctx->stream);
+  if (sch == PRIMA_SPMV_ROW_BLOCKS)
+    return op_spmv_blocks(ctx->precision, a->nblocks, 1, a->ptr, a->col, a->val, a->rbd, x, out,
+                          ctx->stream);
+  return op_spmv_vector(ctx->precision, 1 << (sch - 1), a->rows, a->ptr, a->col, a->val, x, out,
+                        ctx->stream);
+}
+
 // ---------------------------------------------------------------------------
 // FP64 outer iterative refinement (Gleixner, Steffy & Wolter 2016) around
 // float32 device solves of correction LPs on the same matrix.
@@ -1594,6 +1615,17 @@ int prima_mat_upload_csr(prima_ctx* ctx, int32_t rows, int32_t cols, int64_t nnz
     prima_mat_free(ctx, M);
     return dev_fail(ctx, e, "matrix upload");
   }
+  {
+    const std::vector<int32_t> rb = row_blocks(rows, row_ptr);
+    const std::vector<int32_t> d = block_descriptors(rb, row_ptr);
+    M->nblocks = (int32_t)rb.size() - 1;
+    e = hipMalloc((void**)&M->rbd, d.size() * 4);
+    if (e == hipSuccess) e = hipMemcpy(M->rbd, d.data(), d.size() * 4, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      prima_mat_free(ctx, M);
+      return dev_fail(ctx, e, "row-block descriptors");
+    }
+  }
   *out = M;
   return PRIMA_OK;
 }
@@ -1607,6 +1639,7 @@ int prima_mat_free(prima_ctx* ctx, prima_mat* M) {
   if (M->ptr) (void)hipFree(M->ptr);
   if (M->col) (void)hipFree(M->col);
   if (M->val) (void)hipFree(M->val);
+  if (M->rbd) (void)hipFree(M->rbd);
   delete M;
   return PRIMA_OK;
 }
@@ -1617,8 +1650,55 @@ int prima_spmv(prima_ctx* ctx, const prima_mat* a, const prima_vec* x, prima_vec
   if (!a || !x || !out || x->n != a->cols || out->n != a->rows)
     return fail(ctx, PRIMA_EINVAL, "spmv shape mismatch");
   if (x->d == out->d) return fail(ctx, PRIMA_EINVAL, "spmv output aliases its input");
-  HIP_TRY(ctx, op_spmv(ctx->precision, a->rows, a->nnz, a->ptr, a->col, a->val, x->d, out->d,
-                       ctx->stream));
+  HIP_TRY(ctx, spmv_scheduled(ctx, a, x->d, out->d, a->schedule));
+  return PRIMA_OK;
+}
+
+int prima_mat_set_schedule(prima_ctx* ctx, prima_mat* a, int32_t schedule) {
+  int rc = check_open(ctx);
+  if (rc) return rc;
+  if (!a || schedule < 0 || schedule > PRIMA_SPMV_ROW_BLOCKS)
+    return fail(ctx, PRIMA_EINVAL, "unknown SpMV schedule");
+  a->schedule = schedule;
+  return PRIMA_OK;
+}
+
+int prima_mat_autotune(prima_ctx* ctx, prima_mat* a, int32_t reps, int32_t* chosen,
+                       double* best_ms) {
+  int rc = check_open(ctx);
+  if (rc) return rc;
+  if (!a || reps <= 0) return fail(ctx, PRIMA_EINVAL, "bad autotune arguments");
+  const size_t s = esize(ctx->precision);
+  void *x = nullptr, *y = nullptr;
+  HIP_TRY(ctx, hipMalloc(&x, std::max<size_t>(1, (size_t)a->cols * s)));
+  HIP_TRY(ctx, hipMalloc(&y, std::max<size_t>(1, (size_t)a->rows * s)));
+  struct Free {
+    void *x, *y;
+    ~Free() {
+      (void)hipFree(x);
+      (void)hipFree(y);
+    }
+  } guard{x, y};
+  HIP_TRY(ctx, hipMemsetAsync(x, 0, std::max<size_t>(1, (size_t)a->cols * s), ctx->stream));
+  double best = INFINITY;
+  int pick = PRIMA_SPMV_ROW_BLOCKS;
+  for (int sch = PRIMA_SPMV_VECTOR_1; sch <= PRIMA_SPMV_ROW_BLOCKS; ++sch) {
+    HIP_TRY(ctx, spmv_scheduled(ctx, a, x, y, sch));  // warm
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    for (int r = 0; r < reps; ++r) HIP_TRY(ctx, spmv_scheduled(ctx, a, x, y, sch));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    const double per = ms / reps;
+    if (per < best) {
+      best = per;
+      pick = sch;
+    }
+  }
+  a->schedule = pick;
+  if (chosen) *chosen = pick;
+  if (best_ms) *best_ms = best;
   return PRIMA_OK;
 }
 

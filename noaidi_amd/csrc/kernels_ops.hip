@@ -6,6 +6,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include "spmv_blocks.h"
+
 namespace prima {
 
 constexpr int OPS_NT = 256;
@@ -29,7 +31,9 @@ __global__ __launch_bounds__(OPS_NT) void k_spmv(int rows, const int32_t* __rest
   for (int r = blockIdx.x * per_block + threadIdx.x / TPR; r < rows;
        r += gridDim.x * per_block) {
     T acc = T(0);
-    for (int p = ptr[r] + li; p < ptr[r + 1]; p += TPR) acc += val[p] * x[col[p]];
+    // val and col are streamed once: non-temporal loads leave the cache to x
+    for (int p = ptr[r] + li; p < ptr[r + 1]; p += TPR)
+      acc += __builtin_nontemporal_load(&val[p]) * x[__builtin_nontemporal_load(&col[p])];
 #pragma unroll
     for (int off = TPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
     if (li == 0) out[r] = acc;
@@ -136,6 +140,75 @@ hipError_t op_spmv(int prec, int rows, int64_t nnz, const int32_t* ptr, const in
                    const void* val, const void* x, void* out, hipStream_t s) {
   return prec == 32 ? spmv_t<float>(rows, nnz, ptr, col, val, x, out, s)
                     : spmv_t<double>(rows, nnz, ptr, col, val, x, out, s);
+}
+
+// Sub-wave-per-row schedule with an explicit width (the autotuner's "vector"
+// candidates): tpr in {1, 2, 4, ..., 64}.
+template <typename T>
+static hipError_t spmv_vec_t(int tpr, int rows, const int32_t* ptr, const int32_t* col,
+                             const void* val, const void* x, void* out, hipStream_t s) {
+  if (rows == 0) return hipSuccess;
+  const T* v = (const T*)val;
+  const T* xv = (const T*)x;
+  T* o = (T*)out;
+  const int g = grid_for((int64_t)rows * tpr);
+  switch (tpr) {
+    case 1: k_spmv<T, 1><<<g, OPS_NT, 0, s>>>(rows, ptr, col, v, xv, o); break;
+    case 2: k_spmv<T, 2><<<g, OPS_NT, 0, s>>>(rows, ptr, col, v, xv, o); break;
+    case 4: k_spmv<T, 4><<<g, OPS_NT, 0, s>>>(rows, ptr, col, v, xv, o); break;
+    case 8: k_spmv<T, 8><<<g, OPS_NT, 0, s>>>(rows, ptr, col, v, xv, o); break;
+    case 16: k_spmv<T, 16><<<g, OPS_NT, 0, s>>>(rows, ptr, col, v, xv, o); break;
+    case 32: k_spmv<T, 32><<<g, OPS_NT, 0, s>>>(rows, ptr, col, v, xv, o); break;
+    default: k_spmv<T, 64><<<g, OPS_NT, 0, s>>>(rows, ptr, col, v, xv, o); break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t op_spmv_vector(int prec, int tpr, int rows, const int32_t* ptr, const int32_t* col,
+                          const void* val, const void* x, void* out, hipStream_t s) {
+  return prec == 32 ? spmv_vec_t<float>(tpr, rows, ptr, col, val, x, out, s)
+                    : spmv_vec_t<double>(tpr, rows, ptr, col, val, x, out, s);
+}
+
+// Row-block (CSR-Adaptive) schedule: the solver's in-kernel SpMV as a plain
+// launch, each workgroup owning `bpw` consecutive row blocks.
+template <typename T>
+struct EpiStore {
+  T* out;
+  struct Ops {};
+  __device__ Ops load(int) const { return Ops{}; }
+  __device__ void apply(int r, T s, const Ops&) { out[r] = s; }
+};
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_spmv_blocks(const int32_t* __restrict__ ptr,
+                                                    const int32_t* __restrict__ col,
+                                                    const T* __restrict__ val,
+                                                    const T* __restrict__ x,
+                                                    const int32_t* __restrict__ rbd, int nblocks,
+                                                    int bpw, T* __restrict__ out) {
+  __shared__ double tile[TILE];
+  __shared__ double rsum[ROWCAP];
+  __shared__ int rptr[ROWCAP + 1];
+  const int b0 = blockIdx.x * bpw;
+  const int b1 = b0 + bpw < nblocks ? b0 + bpw : nblocks;
+  EpiStore<T> epi{out};
+  spmv_blocks(ptr, col, val, x, rbd, b0, b1, reinterpret_cast<T*>(tile),
+              reinterpret_cast<T*>(rsum), rptr, epi);
+}
+
+hipError_t op_spmv_blocks(int prec, int nblocks, int bpw, const int32_t* ptr, const int32_t* col,
+                          const void* val, const int32_t* rbd, const void* x, void* out,
+                          hipStream_t s) {
+  if (nblocks == 0) return hipSuccess;
+  const int g = (nblocks + bpw - 1) / bpw;
+  if (prec == 32)
+    k_spmv_blocks<float><<<g, NT, 0, s>>>(ptr, col, (const float*)val, (const float*)x, rbd,
+                                          nblocks, bpw, (float*)out);
+  else
+    k_spmv_blocks<double><<<g, NT, 0, s>>>(ptr, col, (const double*)val, (const double*)x, rbd,
+                                           nblocks, bpw, (double*)out);
+  return hipGetLastError();
 }
 
 hipError_t op_axpby(int prec, int64_t n, double a, const void* x, double b, const void* y,

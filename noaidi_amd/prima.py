@@ -133,7 +133,7 @@ EXPORTED = [
     "prima_scale", "prima_copy", "prima_dot", "prima_squared_norm", "prima_primal_step",
     "prima_dual_step", "prima_default_params", "prima_default_mixed_params", "prima_solve",
     "prima_mixed_solve", "prima_solve_batch", "prima_cancel", "prima_stats_get",
-    "prima_stats_reset",
+    "prima_stats_reset", "prima_mat_set_schedule", "prima_mat_autotune",
 ]
 
 _lib = None
@@ -184,6 +184,8 @@ def lib():
             "prima_cancel": (i32, [vp]),
             "prima_stats_get": (i32, [vp, C.POINTER(_Stats)]),
             "prima_stats_reset": (i32, [vp]),
+            "prima_mat_set_schedule": (i32, [vp, vp, i32]),
+            "prima_mat_autotune": (i32, [vp, vp, i32, C.POINTER(i32), C.POINTER(d)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -633,6 +635,16 @@ class HipKernels:
         self._check(lib().prima_mat_upload_csr(self._h, rows, cols, len(va), _ptr(rp), _ptr(ci),
                                                _ptr(va), C.byref(h)), "uploadMatrix")
         return DeviceMat(self, h, rows, cols)
+
+    def set_schedule(self, a: "DeviceMat", schedule: int) -> None:
+        self._check(lib().prima_mat_set_schedule(self._h, a._h, schedule), "set_schedule")
+
+    def autotune(self, a: "DeviceMat", reps: int = 5):
+        """Time every SpMV schedule on `a`; keeps and returns (schedule, ms)."""
+        ch, ms = C.c_int32(), C.c_double()
+        self._check(lib().prima_mat_autotune(self._h, a._h, reps, C.byref(ch), C.byref(ms)),
+                    "autotune")
+        return ch.value, ms.value
 
     def length(self, v: "DeviceVec") -> int:
         return int(lib().prima_vec_length(v._h))
