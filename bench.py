@@ -150,9 +150,9 @@ def spmv_roofline(reps: int = 20):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=32)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--wg", type=int, default=0, help="workgroups per LP (0: tuner)")
     ap.add_argument("--unbatched", type=int, default=4,
                     help="scenarios solved one at a time for the FP64 denominator")

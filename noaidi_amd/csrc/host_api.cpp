@@ -544,7 +544,7 @@ void choose_groups(const prima_params* p, std::vector<SolveJob>& jobs, int capac
   auto ceil_div = [](int64_t a, int64_t b) { return (a + b - 1) / b; };
   int64_t total = 0;
   for (auto& J : jobs) {
-    const int64_t g = ceil_div(std::max<int64_t>(J.h.nnz, 1), tile_nnz() / 2);
+    const int64_t g = ceil_div(std::max<int64_t>(J.h.nnz, 1), 2048);
     J.h.G = (int)std::max<int64_t>(1, std::min<int64_t>(g, capacity));
     J.h.G_cap = std::max(J.h.G_cap, J.h.G);
     total += J.h.G;
@@ -552,7 +552,7 @@ void choose_groups(const prima_params* p, std::vector<SolveJob>& jobs, int capac
   if (total <= capacity) return;
   const int64_t share = std::max<int64_t>(1, capacity / (int64_t)jobs.size());
   for (auto& J : jobs) {
-    const int64_t g = ceil_div(std::max<int64_t>(J.h.nnz, 1), 4 * (int64_t)tile_nnz());
+    const int64_t g = ceil_div(std::max<int64_t>(J.h.nnz, 1), 16384);
     J.h.G = (int)std::max<int64_t>(1, std::min(g, share));
   }
 }

@@ -8,8 +8,9 @@ namespace prima {
 
 constexpr int NT = 512;       // threads per workgroup
 constexpr int NWAVE = NT / 64;
-constexpr int TILE = 4096;    // nnz of a streamed row block (staged in LDS)
-constexpr int ROWCAP = 2048;  // rows of a streamed row block
+constexpr int TILE = 8192;    // nnz of a streamed row block (staged in LDS)
+constexpr int ROWCAP = 4096;  // rows of a streamed row block
+constexpr int DEPTH = 16;     // independent product loads in flight per thread
 
 // ---------------------------------------------------------------------------
 // CSR SpMV over a workgroup's row blocks (CSR-Adaptive style). A block of
@@ -51,17 +52,29 @@ __device__ inline void spmv_blocks(const int32_t* __restrict__ ptr, const int32_
     }
     for (int r = threadIdx.x; r <= nrows; r += NT) rptr[r] = ptr[r0 + r] - p0;
     int k = threadIdx.x;
-    for (; k + 7 * NT < nnz; k += 8 * NT) {
-      int c[8];
-      T v[8], x[8];
+    for (; k + (DEPTH - 1) * NT < nnz; k += DEPTH * NT) {
+      int c[DEPTH];
+      T v[DEPTH], x[DEPTH];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) c[j] = __builtin_nontemporal_load(&col[p0 + k + j * NT]);
+      for (int j = 0; j < DEPTH; ++j) c[j] = __builtin_nontemporal_load(&col[p0 + k + j * NT]);
 #pragma unroll
-      for (int j = 0; j < 8; ++j) v[j] = __builtin_nontemporal_load(&val[p0 + k + j * NT]);
+      for (int j = 0; j < DEPTH; ++j) v[j] = __builtin_nontemporal_load(&val[p0 + k + j * NT]);
 #pragma unroll
-      for (int j = 0; j < 8; ++j) x[j] = xv[c[j]];
+      for (int j = 0; j < DEPTH; ++j) x[j] = xv[c[j]];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) tile[k + j * NT] = v[j] * x[j];
+      for (int j = 0; j < DEPTH; ++j) tile[k + j * NT] = v[j] * x[j];
+    }
+    for (; k + 3 * NT < nnz; k += 4 * NT) {
+      int c[4];
+      T v[4], x[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) c[j] = col[p0 + k + j * NT];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) v[j] = val[p0 + k + j * NT];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) x[j] = xv[c[j]];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) tile[k + j * NT] = v[j] * x[j];
     }
     for (; k + NT < nnz; k += 2 * NT) {
       const int c0 = col[p0 + k], c1 = col[p0 + k + NT];
