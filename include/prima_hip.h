@@ -265,6 +265,62 @@ int prima_solve_batch(prima_ctx* ctx, int32_t count, const prima_lp* lps,
                       const prima_warm* warms, prima_result* results, prima_mixed_info* infos,
                       double** x_outs, double** y_outs, double** rc_outs);
 
+/* ---- branch-and-bound over device relaxations ---------------------------- */
+
+/* MilpStatus (BranchAndBound.scala:7-31), same ordinals. */
+#define PRIMA_MILP_OPTIMAL 0
+#define PRIMA_MILP_FEASIBLE 1
+#define PRIMA_MILP_INFEASIBLE 2
+#define PRIMA_MILP_UNBOUNDED 3
+#define PRIMA_MILP_NO_SOLUTION 4
+
+/* BnbParams (BranchAndBound.scala:57-108) plus `frontier`: up to this many
+ * open nodes are popped together and their relaxations solved in one batched
+ * device call (they share K and differ only in bounds). frontier = 1 is the
+ * reference's sequential depth-first search. mixed may be NULL (plain solves
+ * in the context's precision). */
+typedef struct {
+  prima_params lp;
+  double integrality_tolerance;
+  double gap_abs;
+  double gap_rel;
+  double pruning_safety_factor;
+  int32_t max_nodes;
+  int32_t warm_start;
+  int64_t time_limit_ms; /* < 0: none */
+  int32_t frontier;
+  int32_t pad;
+} prima_bnb_params;
+
+/* MilpSolution scalars (BranchAndBound.scala:33-55). has_incumbent = 0 is the
+ * reference's empty `primal`. relaxations counts LPs solved, which exceeds
+ * nodes_explored when a node solved in a batch was skipped afterwards because
+ * an earlier node of the same batch raised the incumbent. */
+typedef struct {
+  int32_t status;
+  int32_t nodes_explored;
+  int32_t unproven_nodes;
+  int32_t has_incumbent;
+  int32_t relaxations;
+  int32_t batches;
+  double objective;
+  double best_bound;
+  double solve_time_ms;
+} prima_milp_result;
+
+void prima_default_bnb_params(prima_bnb_params* p);
+/* BranchAndBound.solveWith(problem, integers, params, kernels)
+ * (BranchAndBound.scala:168-392) with the node relaxations on the device.
+ * x_out (n) receives the incumbent (integer coordinates snapped) when one
+ * exists. */
+int prima_milp_solve(prima_ctx* ctx, const prima_lp* lp, int32_t n_int, const int32_t* integers,
+                     const prima_bnb_params* params, const prima_mixed_params* mixed,
+                     prima_milp_result* res, double* x_out);
+/* BranchAndBound.skipsBeforeSolving (BranchAndBound.scala:407-413). */
+int prima_bnb_skips_before_solving(double safe_bound, double incumbent, int32_t have_incumbent);
+/* BranchAndBound.safeBoundOf (BranchAndBound.scala:422-427). */
+double prima_bnb_safe_bound_of(int32_t conclusive, double bound, double slack);
+
 /* Launch statistics of this context's solver kernel, accumulated since the
  * last reset: launches and their total duration, timed with HIP events on
  * the context's stream. No counterpart in the reference (its only timing is

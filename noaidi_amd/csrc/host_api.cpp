@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <memory>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -46,6 +47,22 @@ hipError_t op_spmv_blocks(int prec, int nblocks, int bpw, const int32_t* ptr, co
 
 using namespace prima;
 
+// Everything derived from the constraint matrix alone: equilibration,
+// transpose, norm bounds and SpMV row blocks. LPs that share K (branch-and-
+// bound nodes, scenario sweeps over one network) share one of these.
+namespace prima {
+struct MatrixPrep {
+  uint64_t key = 0;
+  std::vector<double> aval;  // scaled values, A order
+  std::vector<int32_t> tptr, tcol;
+  std::vector<double> tval;  // scaled transpose
+  std::vector<double> rs, cs;
+  double spec_scaled = 0, spec_orig = 0;
+  std::vector<int32_t> a_rb, t_rb;    // block start rows (blocks + 1)
+  std::vector<int32_t> a_rbd, t_rbd;  // device descriptors
+};
+}  // namespace prima
+
 struct prima_ctx {
   int device = 0;
   int precision = 64;
@@ -60,6 +77,8 @@ struct prima_ctx {
   bool coop_ok = false;
   prima_stats stats{};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // recently prepared matrices, by content (LPs that share K reuse them)
+  std::vector<std::shared_ptr<const prima::MatrixPrep>> prep_cache;
 };
 
 struct prima_vec {
@@ -128,13 +147,9 @@ struct HostLp {
   int32_t n = 0, m = 0, n_eq = 0;
   int64_t nnz = 0;
   const prima_lp* src = nullptr;
-  std::vector<double> aval;  // scaled values, A order
-  std::vector<int32_t> tptr, tcol;
-  std::vector<double> tval;  // scaled transpose
-  std::vector<double> rs, cs, sc, sq, sl, su;
-  double spec_scaled = 0, spec_orig = 0, rhs_norm = 0, obj_norm = 0;
-  std::vector<int32_t> a_rb, t_rb;    // block start rows (blocks + 1)
-  std::vector<int32_t> a_rbd, t_rbd;  // device descriptors
+  std::shared_ptr<const MatrixPrep> M;
+  std::vector<double> sc, sq, sl, su;
+  double rhs_norm = 0, obj_norm = 0;
   int G = 1;      // workgroups owning this LP in the current launch
   int G_cap = 1;  // most workgroups it may ever get (sizes its arrays)
 };
@@ -229,27 +244,28 @@ double norm2(const double* a, int64_t n) {
   return std::sqrt(s);
 }
 
-// Scaling.apply (Scaling.scala:77-155): Ruiz passes then Pock-Chambolle.
-void equilibrate(const prima_lp* lp, const prima_params* p, HostLp& h) {
+// Scaling.apply (Scaling.scala:77-155), the matrix part: Ruiz passes then
+// Pock-Chambolle, giving the scaled values and the row/column factors.
+void equilibrate_matrix(const prima_lp* lp, const prima_params* p, MatrixPrep& M) {
   const int32_t m = lp->m, n = lp->n;
   const int64_t nnz = lp->nnz;
-  h.aval.assign(lp->val, lp->val + nnz);
-  h.rs.assign((size_t)m, 1.0);
-  h.cs.assign((size_t)n, 1.0);
+  M.aval.assign(lp->val, lp->val + nnz);
+  M.rs.assign((size_t)m, 1.0);
+  M.cs.assign((size_t)n, 1.0);
   std::vector<double> rf((size_t)m), cf((size_t)n);
   auto apply = [&]() {
     for (int32_t r = 0; r < m; ++r)
       for (int32_t q = lp->row_ptr[r]; q < lp->row_ptr[r + 1]; ++q)
-        h.aval[q] = h.aval[q] * rf[r] * cf[lp->col[q]];
-    for (int32_t r = 0; r < m; ++r) h.rs[r] *= rf[r];
-    for (int32_t c = 0; c < n; ++c) h.cs[c] *= cf[c];
+        M.aval[q] = M.aval[q] * rf[r] * cf[lp->col[q]];
+    for (int32_t r = 0; r < m; ++r) M.rs[r] *= rf[r];
+    for (int32_t c = 0; c < n; ++c) M.cs[c] *= cf[c];
   };
   for (int it = 0; it < p->ruiz_iterations; ++it) {
     std::fill(cf.begin(), cf.end(), 0.0);
     for (int32_t r = 0; r < m; ++r) {
       double mx = 0.0;
       for (int32_t q = lp->row_ptr[r]; q < lp->row_ptr[r + 1]; ++q) {
-        const double a = std::fabs(h.aval[q]);
+        const double a = std::fabs(M.aval[q]);
         mx = std::max(mx, a);
         double& cm = cf[lp->col[q]];
         if (a > cm) cm = a;
@@ -265,7 +281,7 @@ void equilibrate(const prima_lp* lp, const prima_params* p, HostLp& h) {
     for (int32_t r = 0; r < m; ++r) {
       double s = 0.0;
       for (int32_t q = lp->row_ptr[r]; q < lp->row_ptr[r + 1]; ++q) {
-        const double a = std::fabs(h.aval[q]);
+        const double a = std::fabs(M.aval[q]);
         s += alpha == 1.0 ? a : std::pow(a, 2.0 - alpha);
         cf[lp->col[q]] += alpha == 1.0 ? a : std::pow(a, alpha);
       }
@@ -274,17 +290,22 @@ void equilibrate(const prima_lp* lp, const prima_params* p, HostLp& h) {
     for (int32_t c = 0; c < n; ++c) cf[c] = cf[c] > 0.0 ? 1.0 / std::sqrt(cf[c]) : 1.0;
     apply();
   }
+}
+
+// Scaling.apply, the vector part: c~ = Dc c, q~ = Dr q, l~ = l / Dc, u~ = u / Dc.
+void scale_vectors(const prima_lp* lp, const MatrixPrep& M, HostLp& h) {
+  const int32_t m = lp->m, n = lp->n;
   h.sc.resize(n);
   h.sl.resize(n);
   h.su.resize(n);
   h.sq.resize(m);
   for (int32_t i = 0; i < n; ++i) {
-    const double d = h.cs[i];
+    const double d = M.cs[i];
     h.sc[i] = lp->c[i] * d;
     h.sl[i] = lp->l[i] / d;
     h.su[i] = lp->u[i] / d;
   }
-  for (int32_t r = 0; r < m; ++r) h.sq[r] = lp->q[r] * h.rs[r];
+  for (int32_t r = 0; r < m; ++r) h.sq[r] = lp->q[r] * M.rs[r];
 }
 
 // Row blocks for the streamed SpMV: consecutive rows while their nnz fit the
@@ -342,32 +363,45 @@ std::vector<int32_t> block_descriptors(const std::vector<int32_t>& rb, const int
   return d;
 }
 
-void prepare(const prima_lp* lp, const prima_params* p, HostLp& h) {
-  h.src = lp;
-  h.n = lp->n;
-  h.m = lp->m;
-  h.n_eq = lp->n_eq;
-  h.nnz = lp->nnz;
+// Content hash of a matrix and the scaling parameters (FNV-1a over the CSR
+// bytes): the cache below reuses a preparation only for identical data.
+uint64_t matrix_key(const prima_lp* lp, const prima_params* p) {
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&](const void* data, size_t bytes) {
+    const unsigned char* b = (const unsigned char*)data;
+    for (size_t i = 0; i < bytes; ++i) h = (h ^ b[i]) * 1099511628211ull;
+  };
+  const int32_t dims[3] = {lp->n, lp->m, p->ruiz_iterations};
+  mix(dims, sizeof(dims));
+  mix(&p->apply_pock_chambolle, sizeof(p->apply_pock_chambolle));
+  mix(&p->pock_chambolle_alpha, sizeof(p->pock_chambolle_alpha));
+  if (lp->m > 0) mix(lp->row_ptr, ((size_t)lp->m + 1) * 4);
+  mix(lp->col, (size_t)lp->nnz * 4);
+  mix(lp->val, (size_t)lp->nnz * 8);
+  return h;
+}
+
+std::shared_ptr<const MatrixPrep> prepare_matrix(const prima_lp* lp, const prima_params* p) {
+  auto M = std::make_shared<MatrixPrep>();
   static const int32_t zero_ptr[1] = {0};
   const int32_t* ptr = lp->m > 0 ? lp->row_ptr : zero_ptr;
-  equilibrate(lp, p, h);
-  transpose_csr(lp->m, lp->n, ptr, lp->col, h.aval.data(), h.tptr, h.tcol, h.tval);
+  equilibrate_matrix(lp, p, *M);
+  transpose_csr(lp->m, lp->n, ptr, lp->col, M->aval.data(), M->tptr, M->tcol, M->tval);
   // spectralNormBound = sqrt(||K||_1 ||K||_inf) (SparseMatrix.scala:181)
-  h.spec_scaled = std::sqrt(norm_inf(lp->n, h.tptr.data(), h.tval.data()) *
-                            norm_inf(lp->m, ptr, h.aval.data()));
+  M->spec_scaled = std::sqrt(norm_inf(lp->n, M->tptr.data(), M->tval.data()) *
+                             norm_inf(lp->m, ptr, M->aval.data()));
   {
     std::vector<int32_t> optr, ocol;
     std::vector<double> oval;
     transpose_csr(lp->m, lp->n, ptr, lp->col, lp->val, optr, ocol, oval);
-    h.spec_orig = std::sqrt(norm_inf(lp->n, optr.data(), oval.data()) *
-                            norm_inf(lp->m, ptr, lp->val));
+    M->spec_orig = std::sqrt(norm_inf(lp->n, optr.data(), oval.data()) *
+                             norm_inf(lp->m, ptr, lp->val));
   }
-  h.rhs_norm = norm2(lp->q, lp->m);
-  h.obj_norm = norm2(lp->c, lp->n);
-  h.a_rb = row_blocks(lp->m, ptr);
-  h.t_rb = row_blocks(lp->n, h.tptr.data());
-  h.a_rbd = block_descriptors(h.a_rb, ptr);
-  h.t_rbd = block_descriptors(h.t_rb, h.tptr.data());
+  M->a_rb = row_blocks(lp->m, ptr);
+  M->t_rb = row_blocks(lp->n, M->tptr.data());
+  M->a_rbd = block_descriptors(M->a_rb, ptr);
+  M->t_rbd = block_descriptors(M->t_rb, M->tptr.data());
+  return M;
 }
 
 // Certificates.classify (Certificates.scala:207-219) in double precision on
@@ -468,20 +502,29 @@ struct Wave {
   int grid = 0;
 };
 
-LpLayout carve(Carver& cv, const HostLp& h, int prec, bool with_ylo) {
+// Sub-array offsets for one LP. With `owner`, the matrix arrays (CSR of A
+// and of its transpose, block descriptors, scale factors) are those of an
+// earlier LP of the wave with the same prepared matrix.
+LpLayout carve(Carver& cv, const HostLp& h, int prec, bool with_ylo, const LpLayout* owner) {
   const size_t s = esize(prec), n = (size_t)h.n, m = (size_t)h.m, z = (size_t)h.nnz;
   const size_t G = (size_t)h.G_cap;
   LpLayout L;
-  L.a_ptr = cv.take((m + 1) * 4);
-  L.a_col = cv.take(z * 4);
-  L.t_ptr = cv.take((n + 1) * 4);
-  L.t_col = cv.take(z * 4);
-  L.a_rb = cv.take(h.a_rbd.size() * 4);
-  L.t_rb = cv.take(h.t_rbd.size() * 4);
+  if (owner) {
+    L = *owner;
+  } else {
+    L.a_ptr = cv.take((m + 1) * 4);
+    L.a_col = cv.take(z * 4);
+    L.t_ptr = cv.take((n + 1) * 4);
+    L.t_col = cv.take(z * 4);
+    L.a_rb = cv.take(h.M->a_rbd.size() * 4);
+    L.t_rb = cv.take(h.M->t_rbd.size() * 4);
+    L.a_val = cv.take(z * s);
+    L.t_val = cv.take(z * s);
+    L.rs = cv.take(m * 8);
+    L.cs = cv.take(n * 8);
+  }
   L.a_wrb = cv.take((G + 1) * 4);
   L.t_wrb = cv.take((G + 1) * 4);
-  L.a_val = cv.take(z * s);
-  L.t_val = cv.take(z * s);
   L.c = cv.take(n * s);
   L.l = cv.take(n * s);
   L.u = cv.take(n * s);
@@ -503,11 +546,9 @@ LpLayout carve(Carver& cv, const HostLp& h, int prec, bool with_ylo) {
   L.oc = cv.take(n * 8);
   L.ol = cv.take(n * 8);
   L.ou = cv.take(n * 8);
-  L.cs = cv.take(n * 8);
   L.x_out = cv.take(n * 8);
   L.rc_out = cv.take(n * 8);
   L.oq = cv.take(m * 8);
-  L.rs = cv.take(m * 8);
   L.y_out = cv.take(m * 8);
   L.part = cv.take((2 * G * kIterParts + G * kEvalParts) * 8);
   L.bar = cv.take(16);
@@ -563,7 +604,17 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
   Wave W;
   W.idx = idx;
   Carver cv;
-  for (int j : idx) W.lay.push_back(carve(cv, jobs[j].h, prec, jobs[j].ylo != nullptr));
+  std::vector<int> owner_of(idx.size(), -1);
+  for (size_t k = 0; k < idx.size(); ++k) {
+    const HostLp& h = jobs[idx[k]].h;
+    for (size_t e = 0; e < k; ++e)
+      if (jobs[idx[e]].h.M == h.M && owner_of[e] < 0) {
+        owner_of[k] = (int)e;
+        break;
+      }
+    W.lay.push_back(carve(cv, h, prec, jobs[idx[k]].ylo != nullptr,
+                          owner_of[k] >= 0 ? &W.lay[owner_of[k]] : nullptr));
+  }
   const size_t arena_bytes = cv.off;
   int grid = 0;
   for (int j : idx) grid += jobs[j].h.G;
@@ -611,18 +662,22 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
     const LpLayout& L = W.lay[k];
     static const int32_t zero_ptr[1] = {0};
     const int32_t* aptr = lp->m > 0 ? lp->row_ptr : zero_ptr;
-    std::vector<int32_t> awrb = partition(h.a_rb, aptr, h.G);
-    std::vector<int32_t> twrb = partition(h.t_rb, h.tptr.data(), h.G);
+    std::vector<int32_t> awrb = partition(h.M->a_rb, aptr, h.G);
+    std::vector<int32_t> twrb = partition(h.M->t_rb, h.M->tptr.data(), h.G);
     HIP_TRY(ctx, put(L.a_wrb, awrb.data(), awrb.size() * 4));
     HIP_TRY(ctx, put(L.t_wrb, twrb.data(), twrb.size() * 4));
-    HIP_TRY(ctx, put(L.a_ptr, aptr, ((size_t)h.m + 1) * 4));
-    HIP_TRY(ctx, put(L.a_col, lp->col, (size_t)h.nnz * 4));
-    HIP_TRY(ctx, put(L.t_ptr, h.tptr.data(), ((size_t)h.n + 1) * 4));
-    HIP_TRY(ctx, put(L.t_col, h.tcol.data(), (size_t)h.nnz * 4));
-    HIP_TRY(ctx, put(L.a_rb, h.a_rbd.data(), h.a_rbd.size() * 4));
-    HIP_TRY(ctx, put(L.t_rb, h.t_rbd.data(), h.t_rbd.size() * 4));
-    HIP_TRY(ctx, put_t(L.a_val, h.aval.data(), (size_t)h.nnz));
-    HIP_TRY(ctx, put_t(L.t_val, h.tval.data(), (size_t)h.nnz));
+    if (owner_of[k] < 0) {  // the first LP of the wave with this matrix uploads it
+      HIP_TRY(ctx, put(L.a_ptr, aptr, ((size_t)h.m + 1) * 4));
+      HIP_TRY(ctx, put(L.a_col, lp->col, (size_t)h.nnz * 4));
+      HIP_TRY(ctx, put(L.t_ptr, h.M->tptr.data(), ((size_t)h.n + 1) * 4));
+      HIP_TRY(ctx, put(L.t_col, h.M->tcol.data(), (size_t)h.nnz * 4));
+      HIP_TRY(ctx, put(L.a_rb, h.M->a_rbd.data(), h.M->a_rbd.size() * 4));
+      HIP_TRY(ctx, put(L.t_rb, h.M->t_rbd.data(), h.M->t_rbd.size() * 4));
+      HIP_TRY(ctx, put_t(L.a_val, h.M->aval.data(), (size_t)h.nnz));
+      HIP_TRY(ctx, put_t(L.t_val, h.M->tval.data(), (size_t)h.nnz));
+      HIP_TRY(ctx, put(L.cs, h.M->cs.data(), (size_t)h.n * 8));
+      HIP_TRY(ctx, put(L.rs, h.M->rs.data(), (size_t)h.m * 8));
+    }
     HIP_TRY(ctx, put_t(L.c, h.sc.data(), h.n));
     HIP_TRY(ctx, put_t(L.l, h.sl.data(), h.n));
     HIP_TRY(ctx, put_t(L.u, h.su.data(), h.n));
@@ -630,13 +685,11 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
     HIP_TRY(ctx, put(L.oc, lp->c, (size_t)h.n * 8));
     HIP_TRY(ctx, put(L.ol, lp->l, (size_t)h.n * 8));
     HIP_TRY(ctx, put(L.ou, lp->u, (size_t)h.n * 8));
-    HIP_TRY(ctx, put(L.cs, h.cs.data(), (size_t)h.n * 8));
     HIP_TRY(ctx, put(L.oq, lp->q, (size_t)h.m * 8));
-    HIP_TRY(ctx, put(L.rs, h.rs.data(), (size_t)h.m * 8));
     if (J.ylo) {
       // y >= ylo in original units is y~ >= ylo / rowScale in scaled units
       std::vector<double> ys((size_t)h.m, 0.0);
-      for (int32_t r = h.n_eq; r < h.m; ++r) ys[r] = J.ylo[r] / h.rs[r];
+      for (int32_t r = h.n_eq; r < h.m; ++r) ys[r] = J.ylo[r] / h.M->rs[r];
       HIP_TRY(ctx, put_t(L.ylo, ys.data(), h.m));
       std::vector<double> yo((size_t)h.m, 0.0);
       for (int32_t r = h.n_eq; r < h.m; ++r) yo[r] = J.ylo[r];
@@ -648,14 +701,14 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
     std::vector<double> x0((size_t)h.n, 0.0), y0((size_t)h.m, 0.0);
     const prima_warm* w = J.warm;
     if (w && w->primal)
-      for (int32_t i = 0; i < h.n; ++i) x0[i] = w->primal[i] / h.cs[i];
+      for (int32_t i = 0; i < h.n; ++i) x0[i] = w->primal[i] / h.M->cs[i];
     for (int32_t i = 0; i < h.n; ++i) {
       const double v = x0[i];
       x0[i] = v < h.sl[i] ? h.sl[i] : (v > h.su[i] ? h.su[i] : v);
     }
     if (w && w->dual)
       for (int32_t r = 0; r < h.m; ++r) {
-        const double v = w->dual[r] / h.rs[r];
+        const double v = w->dual[r] / h.M->rs[r];
         y0[r] = (r >= h.n_eq && v < 0.0) ? 0.0 : v;
       }
     HIP_TRY(ctx, put_t(L.x, x0.data(), h.n));
@@ -667,7 +720,7 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
     std::memset(&st, 0, sizeof(st));
     const bool ws = w && std::isfinite(w->step_size) && w->step_size > 0.0;
     const bool ww = w && std::isfinite(w->primal_weight) && w->primal_weight > 0.0;
-    st.step = ws ? w->step_size : (h.spec_scaled > 0.0 ? 1.0 / h.spec_scaled : 1.0);
+    st.step = ws ? w->step_size : (h.M->spec_scaled > 0.0 ? 1.0 / h.M->spec_scaled : 1.0);
     const double cn = norm2(h.sc.data(), h.n), qn = norm2(h.sq.data(), h.m);
     st.weight = ww ? w->primal_weight : ((cn > 0.0 && qn > 0.0) ? cn / qn : 1.0);
     st.err_at_restart = INFINITY;
@@ -728,7 +781,7 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
     v.offset = lp->offset;
     v.rhs_norm = h.rhs_norm;
     v.obj_norm = h.obj_norm;
-    v.spec_orig = h.spec_orig;
+    v.spec_orig = h.M->spec_orig;
     W.views.push_back(v);
     for (int g = 0; g < h.G; ++g) W.wg_lp[wg_base + g] = (int32_t)k;
     wg_base += h.G;
@@ -795,8 +848,8 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
       J.h.G = newG[j];
       static const int32_t zero_ptr[1] = {0};
       const int32_t* aptr = J.lp->m > 0 ? J.lp->row_ptr : zero_ptr;
-      std::vector<int32_t> awrb = partition(J.h.a_rb, aptr, J.h.G);
-      std::vector<int32_t> twrb = partition(J.h.t_rb, J.h.tptr.data(), J.h.G);
+      std::vector<int32_t> awrb = partition(J.h.M->a_rb, aptr, J.h.G);
+      std::vector<int32_t> twrb = partition(J.h.M->t_rb, J.h.M->tptr.data(), J.h.G);
       HIP_TRY(ctx, put(W.lay[k].a_wrb, awrb.data(), awrb.size() * 4));
       HIP_TRY(ctx, put(W.lay[k].t_wrb, twrb.data(), twrb.size() * 4));
       W.views[k].G = J.h.G;
@@ -960,7 +1013,25 @@ int prepare_job(prima_ctx* ctx, SolveJob& J, const prima_lp* lp, const prima_par
   auto t0 = clk::now();
   J.lp = lp;
   J.warm = warm;
-  prepare(lp, p, J.h);
+  HostLp& h = J.h;
+  h.src = lp;
+  h.n = lp->n;
+  h.m = lp->m;
+  h.n_eq = lp->n_eq;
+  h.nnz = lp->nnz;
+  const uint64_t key = matrix_key(lp, p);
+  for (const auto& c : ctx->prep_cache)
+    if (c->key == key) h.M = c;
+  if (!h.M) {
+    auto M = prepare_matrix(lp, p);
+    const_cast<MatrixPrep&>(*M).key = key;
+    ctx->prep_cache.push_back(M);
+    if (ctx->prep_cache.size() > 4) ctx->prep_cache.erase(ctx->prep_cache.begin());
+    h.M = M;
+  }
+  scale_vectors(lp, *h.M, h);
+  h.rhs_norm = norm2(lp->q, lp->m);
+  h.obj_norm = norm2(lp->c, lp->n);
   J.prep_ms = ms_since(t0);
   return PRIMA_OK;
 }
@@ -1390,6 +1461,270 @@ int refine_batch(prima_ctx* ctx, int32_t count, const prima_lp* lps, const prima
   return PRIMA_OK;
 }
 
+
+// ---------------------------------------------------------------------------
+// Branch-and-bound (BranchAndBound.scala:168-469) with a batched frontier.
+
+struct BnbWarm {
+  std::vector<double> x, y;
+  double step = NAN, weight = NAN;
+};
+
+struct BnbNode {
+  std::vector<double> lower, upper;
+  double safe_bound;
+  std::shared_ptr<const BnbWarm> warm;
+};
+
+// skipsBeforeSolving (BranchAndBound.scala:407-413): the negation of >= so a
+// NaN bound falls through to solving.
+bool skips_before_solving(double safe_bound, double incumbent, bool have) {
+  return have && safe_bound >= incumbent;
+}
+
+// safeBoundOf (BranchAndBound.scala:422-427).
+double safe_bound_of(bool conclusive, double bound, double slack) {
+  if (!conclusive) return -INFINITY;
+  const double c = bound - slack;
+  return std::isfinite(c) ? c : -INFINITY;
+}
+
+// Java's Math.round as a double: nearest whole number, ties toward +inf,
+// NaN -> 0 (BranchAndBound.scala:447 and :461 call it).
+double java_round(double v) {
+  if (std::isnan(v)) return 0.0;
+  if (!std::isfinite(v) || std::fabs(v) >= 4503599627370496.0) return v;  // 2^52: already whole
+  const double f = std::floor(v);
+  return (v - f >= 0.5) ? f + 1.0 : f;
+}
+
+// mostFractional (BranchAndBound.scala:441-453): -1 when all are integral.
+int most_fractional(const double* x, const std::vector<int32_t>& ints, double tol) {
+  int best = -1;
+  double worst = tol;
+  for (int32_t j : ints) {
+    const double g = std::fabs(x[j] - java_round(x[j]));
+    if (g > worst) {
+      worst = g;
+      best = j;
+    }
+  }
+  return best;
+}
+
+}  // namespace
+
+extern "C" void prima_default_bnb_params(prima_bnb_params* p) {
+  std::memset(p, 0, sizeof(*p));
+  prima_default_params(&p->lp);
+  p->integrality_tolerance = 1e-6;
+  p->gap_abs = 1e-6;
+  p->gap_rel = 1e-6;
+  p->pruning_safety_factor = 10.0;
+  p->max_nodes = 100000;
+  p->warm_start = 1;
+  p->time_limit_ms = -1;
+  p->frontier = 1;
+}
+
+extern "C" int prima_bnb_skips_before_solving(double safe_bound, double incumbent,
+                                              int32_t have_incumbent) {
+  return skips_before_solving(safe_bound, incumbent, have_incumbent != 0) ? 1 : 0;
+}
+
+extern "C" double prima_bnb_safe_bound_of(int32_t conclusive, double bound, double slack) {
+  return safe_bound_of(conclusive != 0, bound, slack);
+}
+
+extern "C" int prima_milp_solve(prima_ctx* ctx, const prima_lp* lp, int32_t n_int,
+                                const int32_t* integers, const prima_bnb_params* bp,
+                                const prima_mixed_params* mixed, prima_milp_result* res,
+                                double* x_out) {
+  int rc = check_open(ctx);
+  if (rc) return rc;
+  if (!lp || !bp || !res || n_int < 0 || (n_int > 0 && !integers))
+    return fail(ctx, PRIMA_EINVAL, "bad branch-and-bound arguments");
+  std::string e = validate_lp(lp);
+  if (!e.empty()) return fail(ctx, PRIMA_EINVAL, e);
+  for (int32_t k = 0; k < n_int; ++k)
+    if (integers[k] < 0 || integers[k] >= lp->n)
+      return fail(ctx, PRIMA_EINVAL, "integer column " + std::to_string(integers[k]) +
+                                         " is out of range");
+  if (bp->frontier < 1 || bp->max_nodes < 1) return fail(ctx, PRIMA_EINVAL, "bad frontier");
+  const auto t0 = clk::now();
+  std::memset(res, 0, sizeof(*res));
+  const int32_t n = lp->n, m = lp->m;
+  auto solve_nodes = [&](std::vector<prima_lp>& subs, std::vector<prima_warm>& warms,
+                         std::vector<prima_result>& out, std::vector<std::vector<double>>& xs,
+                         std::vector<std::vector<double>>& ys) -> int {
+    const int32_t k = (int32_t)subs.size();
+    out.assign(k, prima_result{});
+    xs.assign(k, std::vector<double>((size_t)n));
+    ys.assign(k, std::vector<double>((size_t)m));
+    std::vector<double*> px(k), py(k), prc(k, nullptr);
+    for (int32_t i = 0; i < k; ++i) {
+      px[i] = xs[i].data();
+      py[i] = ys[i].data();
+    }
+    std::vector<prima_mixed_info> infos(k);
+    res->batches++;
+    res->relaxations += k;
+    return prima_solve_batch(ctx, k, subs.data(), &bp->lp, mixed, warms.data(), out.data(),
+                             mixed ? infos.data() : nullptr, px.data(), py.data(), prc.data());
+  };
+
+  if (n_int == 0) {  // BranchAndBound.scala:181-207: the relaxation is the answer
+    std::vector<prima_lp> subs{*lp};
+    std::vector<prima_warm> warms{prima_warm{nullptr, nullptr, NAN, NAN}};
+    std::vector<prima_result> out;
+    std::vector<std::vector<double>> xs, ys;
+    rc = solve_nodes(subs, warms, out, xs, ys);
+    if (rc) return rc;
+    const int st = out[0].status;
+    res->status = st == PRIMA_OPTIMAL            ? PRIMA_MILP_OPTIMAL
+                  : st == PRIMA_PRIMAL_INFEASIBLE ? PRIMA_MILP_INFEASIBLE
+                  : st == PRIMA_DUAL_INFEASIBLE   ? PRIMA_MILP_UNBOUNDED
+                                                  : PRIMA_MILP_NO_SOLUTION;
+    const bool solved = res->status == PRIMA_MILP_OPTIMAL;
+    res->objective = solved ? out[0].objective : INFINITY;
+    res->best_bound = solved ? out[0].objective : -INFINITY;
+    res->nodes_explored = 1;
+    res->unproven_nodes = conclusive(st) ? 0 : 1;
+    res->has_incumbent = solved ? 1 : 0;
+    if (solved && x_out) std::memcpy(x_out, xs[0].data(), (size_t)n * 8);
+    res->solve_time_ms = ms_since(t0);
+    return PRIMA_OK;
+  }
+
+  std::vector<int32_t> ordered(integers, integers + n_int);
+  std::sort(ordered.begin(), ordered.end());
+  ordered.erase(std::unique(ordered.begin(), ordered.end()), ordered.end());
+  std::vector<BnbNode> stack;
+  stack.push_back(BnbNode{std::vector<double>(lp->l, lp->l + n),
+                          std::vector<double>(lp->u, lp->u + n), -INFINITY, nullptr});
+  std::vector<double> incumbent;
+  double inc_val = INFINITY;
+  int nodes = 0, unproven = 0;
+  bool root_unbounded = false;
+  auto out_of_time = [&]() {
+    return bp->time_limit_ms >= 0 && ms_since(t0) > (double)bp->time_limit_ms;
+  };
+  while (!stack.empty() && nodes < bp->max_nodes && !out_of_time()) {
+    // pop the frontier, applying the pre-solve skip per node as the
+    // sequential search would at its pop
+    std::vector<BnbNode> batch;
+    while (!stack.empty() && (int)batch.size() < bp->frontier &&
+           nodes + (int)batch.size() < bp->max_nodes) {
+      BnbNode node = std::move(stack.back());
+      stack.pop_back();
+      if (skips_before_solving(node.safe_bound, inc_val, !incumbent.empty())) continue;
+      batch.push_back(std::move(node));
+    }
+    if (batch.empty()) continue;
+    std::vector<prima_lp> subs(batch.size(), *lp);
+    std::vector<prima_warm> warms(batch.size(), prima_warm{nullptr, nullptr, NAN, NAN});
+    for (size_t i = 0; i < batch.size(); ++i) {
+      subs[i].l = batch[i].lower.data();
+      subs[i].u = batch[i].upper.data();
+      if (bp->warm_start && batch[i].warm) {
+        const BnbWarm& w = *batch[i].warm;
+        warms[i] = prima_warm{w.x.data(), w.y.data(), w.step, w.weight};
+      }
+    }
+    std::vector<prima_result> out;
+    std::vector<std::vector<double>> xs, ys;
+    rc = solve_nodes(subs, warms, out, xs, ys);
+    if (rc) return rc;
+    for (size_t i = 0; i < batch.size(); ++i) {
+      const BnbNode& node = batch[i];
+      const prima_result& rel = out[i];
+      // an earlier node of this batch may have raised the incumbent past this
+      // node's parent bound: the sequential search would not have solved it
+      if (skips_before_solving(node.safe_bound, inc_val, !incumbent.empty())) continue;
+      nodes++;
+      if (rel.status == PRIMA_PRIMAL_INFEASIBLE) continue;
+      if (rel.status == PRIMA_DUAL_INFEASIBLE) {
+        if (nodes == 1) root_unbounded = true;
+        else unproven++;
+        continue;
+      }
+      const bool concl = conclusive(rel.status);
+      if (!concl) unproven++;
+      const double bound = rel.objective;
+      const double slack = bp->pruning_safety_factor *
+                           std::max(std::fabs(rel.objective - rel.dual_objective), bp->lp.eps_abs);
+      const double safe = safe_bound_of(concl, bound, slack);
+      const bool prunable = concl && !incumbent.empty() && safe >= inc_val;
+      if (prunable) continue;
+      const int j = most_fractional(xs[i].data(), ordered, bp->integrality_tolerance);
+      if (j < 0) {
+        if (!concl) continue;  // integral but unconverged: never an incumbent
+        std::vector<double> snapped = xs[i];
+        for (int32_t c : ordered) snapped[c] = java_round(snapped[c]);
+        double value = lp->offset;
+        for (int32_t c = 0; c < n; ++c) value += lp->c[c] * snapped[c];
+        if (value < inc_val) {
+          inc_val = value;
+          incumbent = std::move(snapped);
+        }
+        continue;
+      }
+      std::shared_ptr<const BnbWarm> warm;
+      if (bp->warm_start) {
+        bool ok = true;  // Pdhg.WarmStart's preconditions: finite vectors
+        for (double v : xs[i]) ok &= std::isfinite(v);
+        for (double v : ys[i]) ok &= std::isfinite(v);
+        if (ok) {
+          auto w = std::make_shared<BnbWarm>();
+          w->x = xs[i];
+          w->y = ys[i];
+          w->step = rel.final_step_size;
+          w->weight = rel.final_primal_weight;
+          warm = w;
+        }
+      }
+      const double value = xs[i][j];
+      const double fl = std::floor(value), ce = std::ceil(value);
+      // ceiling first so the floor branch is explored first
+      if (ce <= node.upper[j]) {
+        BnbNode child{node.lower, node.upper, safe, warm};
+        child.lower[j] = std::max(child.lower[j], ce);
+        stack.push_back(std::move(child));
+      }
+      if (fl >= node.lower[j]) {
+        BnbNode child{node.lower, node.upper, safe, warm};
+        child.upper[j] = std::min(child.upper[j], fl);
+        stack.push_back(std::move(child));
+      }
+    }
+  }
+  double open_bound = inc_val;
+  for (const auto& nd : stack) open_bound = std::min(open_bound, nd.safe_bound);
+  const double bound = stack.empty() ? inc_val : open_bound;
+  auto closed = [&](double inc, double b) {
+    const double a = std::fabs(inc - b);
+    return a <= bp->gap_abs || a <= bp->gap_rel * std::max(1.0, std::fabs(inc));
+  };
+  int status;
+  if (root_unbounded) status = PRIMA_MILP_UNBOUNDED;
+  else if (incumbent.empty() && stack.empty() && unproven == 0) status = PRIMA_MILP_INFEASIBLE;
+  else if (incumbent.empty()) status = PRIMA_MILP_NO_SOLUTION;
+  else if (unproven > 0) status = PRIMA_MILP_FEASIBLE;
+  else if (stack.empty()) status = PRIMA_MILP_OPTIMAL;
+  else if (closed(inc_val, bound)) status = PRIMA_MILP_OPTIMAL;
+  else status = PRIMA_MILP_FEASIBLE;
+  res->status = status;
+  res->objective = inc_val;
+  res->best_bound = incumbent.empty() ? bound : std::min(bound, inc_val);
+  res->nodes_explored = nodes;
+  res->unproven_nodes = unproven;
+  res->has_incumbent = incumbent.empty() ? 0 : 1;
+  res->solve_time_ms = ms_since(t0);
+  if (!incumbent.empty() && x_out) std::memcpy(x_out, incumbent.data(), (size_t)n * 8);
+  return PRIMA_OK;
+}
+
+namespace {
 }  // namespace
 
 // ===========================================================================

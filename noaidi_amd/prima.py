@@ -45,6 +45,10 @@ STATUS = [
 ]
 CONCLUSIVE = {"Optimal", "PrimalInfeasible", "DualInfeasible"}
 
+# MilpStatus (BranchAndBound.scala:7-31), same ordinals as the C ABI.
+MILP_STATUS = ["Optimal", "Feasible", "Infeasible", "Unbounded", "NoSolutionFound"]
+MILP_CONCLUSIVE = {"Optimal", "Infeasible", "Unbounded"}
+
 PRIMA_OK, PRIMA_EINVAL, PRIMA_ENOMEM, PRIMA_EDEVICE, PRIMA_ECLOSED = 0, -1, -2, -3, -4
 
 
@@ -115,6 +119,22 @@ class _Stats(C.Structure):
                 ("evaluations", C.c_int64), ("algorithmic_bytes", C.c_double)]
 
 
+class _BnbParams(C.Structure):
+    _fields_ = [("lp", _Params), ("integrality_tolerance", C.c_double),
+                ("gap_abs", C.c_double), ("gap_rel", C.c_double),
+                ("pruning_safety_factor", C.c_double), ("max_nodes", C.c_int32),
+                ("warm_start", C.c_int32), ("time_limit_ms", C.c_int64),
+                ("frontier", C.c_int32), ("pad", C.c_int32)]
+
+
+class _MilpResult(C.Structure):
+    _fields_ = [("status", C.c_int32), ("nodes_explored", C.c_int32),
+                ("unproven_nodes", C.c_int32), ("has_incumbent", C.c_int32),
+                ("relaxations", C.c_int32), ("batches", C.c_int32),
+                ("objective", C.c_double), ("best_bound", C.c_double),
+                ("solve_time_ms", C.c_double)]
+
+
 class _Instance(C.Structure):
     _fields_ = [("n", C.c_int32), ("m", C.c_int32), ("n_eq", C.c_int32), ("pad", C.c_int32),
                 ("nnz", C.c_int64), ("row_ptr", C.POINTER(C.c_int32)),
@@ -134,6 +154,8 @@ EXPORTED = [
     "prima_dual_step", "prima_default_params", "prima_default_mixed_params", "prima_solve",
     "prima_mixed_solve", "prima_solve_batch", "prima_cancel", "prima_stats_get",
     "prima_stats_reset", "prima_mat_set_schedule", "prima_mat_autotune",
+    "prima_default_bnb_params", "prima_milp_solve", "prima_bnb_skips_before_solving",
+    "prima_bnb_safe_bound_of",
 ]
 
 _lib = None
@@ -186,6 +208,11 @@ def lib():
             "prima_stats_reset": (i32, [vp]),
             "prima_mat_set_schedule": (i32, [vp, vp, i32]),
             "prima_mat_autotune": (i32, [vp, vp, i32, C.POINTER(i32), C.POINTER(d)]),
+            "prima_default_bnb_params": (None, [C.POINTER(_BnbParams)]),
+            "prima_milp_solve": (i32, [vp, C.POINTER(_Lp), i32, vp, C.POINTER(_BnbParams),
+                                       C.POINTER(_Mixed), C.POINTER(_MilpResult), vp]),
+            "prima_bnb_skips_before_solving": (i32, [d, d, i32]),
+            "prima_bnb_safe_bound_of": (d, [i32, d, d]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -552,6 +579,73 @@ def _solution(r: _Result, x, y, rc, info: Optional[_MixedInfo] = None) -> LpSolu
     return s
 
 
+@dataclass
+class BnbParams:
+    """BnbParams (BranchAndBound.scala:57-108). `frontier` open nodes are
+    popped together and their relaxations solved in one batched device call;
+    frontier = 1 is the reference's sequential depth-first search."""
+
+    lp: PdhgParams = field(default_factory=PdhgParams)
+    integrality_tolerance: float = 1e-6
+    gap_abs: float = 1e-6
+    gap_rel: float = 1e-6
+    pruning_safety_factor: float = 10.0
+    max_nodes: int = 100_000
+    time_limit_ms: Optional[int] = None
+    warm_start: bool = True
+    frontier: int = 1
+
+    def copy(self, **kw) -> "BnbParams":
+        return replace(self, **kw)
+
+    def _c(self) -> _BnbParams:
+        b = _BnbParams()
+        b.lp = self.lp._c()
+        b.integrality_tolerance = self.integrality_tolerance
+        b.gap_abs, b.gap_rel = self.gap_abs, self.gap_rel
+        b.pruning_safety_factor = self.pruning_safety_factor
+        b.max_nodes = self.max_nodes
+        b.warm_start = 1 if self.warm_start else 0
+        b.time_limit_ms = -1 if self.time_limit_ms is None else int(self.time_limit_ms)
+        b.frontier = self.frontier
+        return b
+
+
+@dataclass
+class MilpSolution:
+    """MilpSolution (BranchAndBound.scala:33-55); `primal` is empty when no
+    integer solution was found."""
+
+    status: str
+    primal: np.ndarray
+    objective_value: float
+    best_bound: float
+    nodes_explored: int
+    unproven_nodes: int
+    solve_time_ms: float
+    relaxations: int = 0
+    batches: int = 0
+
+    @property
+    def gap(self) -> float:
+        return abs(self.objective_value - self.best_bound)
+
+    @property
+    def is_conclusive(self) -> bool:
+        return self.status in MILP_CONCLUSIVE
+
+
+def skips_before_solving(safe_bound: float, incumbent: float, have_incumbent: bool) -> bool:
+    """BranchAndBound.skipsBeforeSolving, evaluated by the library."""
+    return bool(lib().prima_bnb_skips_before_solving(safe_bound, incumbent,
+                                                     1 if have_incumbent else 0))
+
+
+def safe_bound_of(conclusive: bool, bound: float, slack: float) -> float:
+    """BranchAndBound.safeBoundOf, evaluated by the library."""
+    return lib().prima_bnb_safe_bound_of(1 if conclusive else 0, bound, slack)
+
+
 # ---------------------------------------------------------------------------
 # the Kernels seam
 
@@ -734,6 +828,26 @@ class HipKernels:
                 for i in range(k)]
 
 
+    def milp_solve(self, problem: LpProblem, integers, params: BnbParams = BnbParams(),
+                   mixed: Optional[MixedParams] = None) -> MilpSolution:
+        """BranchAndBound.solveWith(problem, integers, params, this): node
+        relaxations on the device, `params.frontier` of them per batched call."""
+        ints = np.ascontiguousarray(sorted(set(int(j) for j in integers)), dtype=np.int32)
+        lp = problem._c()
+        cp = params._c()
+        cm = mixed._c() if mixed is not None else None
+        res = _MilpResult()
+        x = np.zeros(max(problem.n, 1))
+        rc = lib().prima_milp_solve(self._h, C.byref(lp), len(ints), _ptr(ints) if len(ints) else None,
+                                    C.byref(cp), C.byref(cm) if cm is not None else None,
+                                    C.byref(res), _ptr(x))
+        self._check(rc, "milp_solve")
+        primal = x[: problem.n].copy() if res.has_incumbent else np.zeros(0)
+        return MilpSolution(MILP_STATUS[res.status], primal, res.objective, res.best_bound,
+                            res.nodes_explored, res.unproven_nodes, res.solve_time_ms,
+                            res.relaxations, res.batches)
+
+
 class DeviceVec:
     def __init__(self, k: HipKernels, h, n: int):
         self._k, self._h, self.n = k, h, n
@@ -793,6 +907,19 @@ def solve_all(problems: Sequence[LpProblem], params: PdhgParams = PdhgParams(),
     finally:
         if own:
             k.close()
+
+
+def bnb_solve(problem: LpProblem, integers, params: BnbParams = BnbParams(),
+              device: int = 0) -> MilpSolution:
+    """BranchAndBound.solve: one FP64 device backend for the whole search."""
+    with HipKernels(device, 64) as k:
+        return k.milp_solve(problem, integers, params)
+
+
+def bnb_solve_with(problem: LpProblem, integers, params: BnbParams, kernels: HipKernels,
+                   mixed: Optional[MixedParams] = None) -> MilpSolution:
+    """BranchAndBound.solveWith against a caller-owned backend."""
+    return kernels.milp_solve(problem, integers, params, mixed)
 
 
 class HipPdhgSolver:

@@ -338,3 +338,166 @@ class JRandom:
 
 def fdlibm_log(x: float) -> float:
     return lib().orc_fdlibm_log(x)
+
+
+# ---------------------------------------------------------------------------
+# Branch-and-bound (BranchAndBound.scala:168-469), sequential depth-first, with
+# every node relaxation solved by the oracle's FP64 PDHG.
+
+MILP_STATUS_NAMES = ["Optimal", "Feasible", "Infeasible", "Unbounded", "NoSolutionFound"]
+_CONCLUSIVE = {"Optimal", "PrimalInfeasible", "DualInfeasible"}
+
+
+@dataclass
+class MilpSolution:
+    status: str
+    primal: np.ndarray
+    objective: float
+    best_bound: float
+    nodes_explored: int
+    unproven_nodes: int
+
+
+def java_round(v: float) -> float:
+    """Math.round(v).toDouble: nearest whole number, ties toward +inf, NaN -> 0."""
+    if v != v:
+        return 0.0
+    if not np.isfinite(v) or abs(v) >= 2.0 ** 52:
+        return v
+    f = float(np.floor(v))
+    return f + 1.0 if v - f >= 0.5 else f
+
+
+def skips_before_solving(safe_bound: float, incumbent: float, have: bool) -> bool:
+    """BranchAndBound.scala:407-413."""
+    return have and safe_bound >= incumbent
+
+
+def safe_bound_of(conclusive: bool, bound: float, slack: float) -> float:
+    """BranchAndBound.scala:422-427."""
+    if not conclusive:
+        return float("-inf")
+    c = bound - slack
+    return c if np.isfinite(c) else float("-inf")
+
+
+def most_fractional(x, integers, tol):
+    """BranchAndBound.scala:441-453: the furthest from whole, first on ties."""
+    best, worst = -1, tol
+    for j in integers:
+        g = abs(x[j] - java_round(x[j]))
+        if g > worst:
+            worst, best = g, j
+    return best
+
+
+def _node_lp(data, lower, upper) -> OracleLp:
+    return OracleLp.from_csr(data["n"], data["m"], data["n_eq"], data["row_ptr"], data["col"],
+                             data["val"], data["c"], data["q"], lower, upper, data["offset"])
+
+
+def _primal_objective(data, x) -> float:
+    """LpProblem.primalObjective (LpProblem.scala:42-50): offset first, then in order."""
+    s = data["offset"]
+    for ci, xi in zip(data["c"], x):
+        s += ci * xi
+    return s
+
+
+def bnb_solve(lp: OracleLp, integers, params: Params | None = None,
+              integrality_tolerance=1e-6, gap_abs=1e-6, gap_rel=1e-6,
+              pruning_safety_factor=10.0, max_nodes=100_000, warm_start=True) -> MilpSolution:
+    """BranchAndBound.solveWith (BranchAndBound.scala:168-392), no time limit."""
+    params = params or default_params()
+    data = lp.export()
+    n = data["n"]
+    for j in integers:
+        if not (0 <= j < n):
+            raise ValueError(f"integer column {j} is out of range [0, {n})")
+    if not integers:  # :181-207
+        root = lp.solve(params)
+        st = {"Optimal": "Optimal", "PrimalInfeasible": "Infeasible",
+              "DualInfeasible": "Unbounded"}.get(root.status, "NoSolutionFound")
+        ok = st == "Optimal"
+        return MilpSolution(st, root.primal if ok else np.zeros(0),
+                            root.objective if ok else float("inf"),
+                            root.objective if ok else float("-inf"), 1,
+                            0 if root.status in _CONCLUSIVE else 1)
+    ordered = sorted(set(integers))
+    stack = [(data["l"].copy(), data["u"].copy(), float("-inf"), None)]
+    incumbent, inc_val = None, float("inf")
+    nodes = unproven = 0
+    root_unbounded = False
+    while stack and nodes < max_nodes:
+        lower, upper, safe_parent, warm = stack.pop()
+        if skips_before_solving(safe_parent, inc_val, incumbent is not None):
+            continue
+        sub = _node_lp(data, lower, upper)
+        if warm_start and warm is not None:
+            rel = sub.solve(params, 64, warm[0], warm[1], warm[2], warm[3])
+        else:
+            rel = sub.solve(params)
+        nodes += 1
+        if rel.status == "PrimalInfeasible":
+            continue
+        if rel.status == "DualInfeasible":
+            if nodes == 1:
+                root_unbounded = True
+            else:
+                unproven += 1
+            continue
+        conclusive = rel.status in _CONCLUSIVE
+        if not conclusive:
+            unproven += 1
+        slack = pruning_safety_factor * max(abs(rel.objective - rel.dual_objective),
+                                            params.eps_abs)
+        safe = safe_bound_of(conclusive, rel.objective, slack)
+        if conclusive and incumbent is not None and safe >= inc_val:
+            continue
+        j = most_fractional(rel.primal, ordered, integrality_tolerance)
+        if j < 0:
+            if conclusive:
+                snapped = rel.primal.copy()
+                for c in ordered:
+                    snapped[c] = java_round(snapped[c])
+                value = _primal_objective(data, snapped)
+                if value < inc_val:
+                    inc_val, incumbent = value, snapped
+            continue
+        child_warm = None
+        if warm_start and np.all(np.isfinite(rel.primal)) and np.all(np.isfinite(rel.dual)):
+            ok = lambda v: v if (np.isfinite(v) and v > 0) else float("nan")
+            child_warm = (rel.primal.copy(), rel.dual.copy(), ok(rel.final_step_size),
+                          ok(rel.final_primal_weight))
+        v = rel.primal[j]
+        fl, ce = float(np.floor(v)), float(np.ceil(v))
+        if ce <= upper[j]:
+            lo = lower.copy()
+            lo[j] = max(lo[j], ce)
+            stack.append((lo, upper, safe, child_warm))
+        if fl >= lower[j]:
+            up = upper.copy()
+            up[j] = min(up[j], fl)
+            stack.append((lower, up, safe, child_warm))
+    open_bound = inc_val if not stack else min(inc_val, min(s[2] for s in stack))
+    bound = inc_val if not stack else open_bound
+
+    def closed(inc, b):
+        a = abs(inc - b)
+        return a <= gap_abs or a <= gap_rel * max(1.0, abs(inc))
+
+    have = incumbent is not None
+    if root_unbounded:
+        st = "Unbounded"
+    elif not have and not stack and unproven == 0:
+        st = "Infeasible"
+    elif not have:
+        st = "NoSolutionFound"
+    elif unproven > 0:
+        st = "Feasible"
+    elif not stack or closed(inc_val, bound):
+        st = "Optimal"
+    else:
+        st = "Feasible"
+    return MilpSolution(st, incumbent if have else np.zeros(0), inc_val,
+                        bound if not have else min(bound, inc_val), nodes, unproven)

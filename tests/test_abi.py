@@ -56,12 +56,41 @@ def test_defaults_match_reference():
         assert getattr(q, f) == getattr(p, f), f
 
 
+def test_bnb_defaults_match_reference():
+    # BnbParams (BranchAndBound.scala:57-108); frontier 1 is the sequential search
+    b = prima._BnbParams()
+    prima.lib().prima_default_bnb_params(C.byref(b))
+    assert (b.integrality_tolerance, b.gap_abs, b.gap_rel, b.pruning_safety_factor) == (
+        1e-6, 1e-6, 1e-6, 10.0)
+    assert (b.max_nodes, b.warm_start, b.time_limit_ms, b.frontier) == (100000, 1, -1, 1)
+    assert b.lp.eps_abs == 1e-8 and b.lp.max_iterations == 100000
+    q = prima.BnbParams()._c()
+    for f, _ in prima._BnbParams._fields_:
+        if f != "lp":
+            assert getattr(q, f) == getattr(b, f), f
+
+
+def test_bnb_predicates_in_the_library():
+    # BranchAndBoundSuite.scala:130-158, evaluated by the C++ driver's own helpers
+    nan, inf = float("nan"), float("inf")
+    assert not prima.skips_before_solving(nan, 10.0, True)
+    assert not prima.skips_before_solving(-inf, 10.0, True)
+    assert prima.skips_before_solving(11.0, 10.0, True)
+    assert not prima.skips_before_solving(9.0, 10.0, True)
+    assert not prima.skips_before_solving(11.0, inf, False)
+    assert prima.safe_bound_of(True, 5.0, 1.0) == 4.0
+    assert prima.safe_bound_of(True, nan, 1.0) == -inf
+    assert prima.safe_bound_of(True, 5.0, nan) == -inf
+    assert prima.safe_bound_of(False, 5.0, 1.0) == -inf
+
+
 def test_struct_layouts_match_header(tmp_path):
     # compile the header and compare every field offset with the ctypes mirror
     structs = {"prima_lp": prima._Lp, "prima_params": prima._Params,
                "prima_result": prima._Result, "prima_warm": prima._Warm,
                "prima_mixed_params": prima._Mixed, "prima_mixed_info": prima._MixedInfo,
-               "prima_stats": prima._Stats, "prima_capabilities": prima._Caps}
+               "prima_stats": prima._Stats, "prima_capabilities": prima._Caps,
+               "prima_bnb_params": prima._BnbParams, "prima_milp_result": prima._MilpResult}
     lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "prima_hip.h"',
              "int main(void) {"]
     for cname, py in structs.items():

@@ -1,0 +1,147 @@
+"""Branch-and-bound with node relaxations on the device (prima_milp_solve)
+against BranchAndBoundSuite's hand-worked optima, brute-force enumeration and
+the oracle's sequential search. frontier = 1 is the reference's depth-first
+search node for node; larger frontiers solve that many open nodes per batched
+device call and must reach the same optimum."""
+import math
+
+import numpy as np
+import pytest
+
+from noaidi_amd import prima
+from oracle import oracle as orc
+from tests import milp_fixtures as mf
+
+pytestmark = pytest.mark.gpu
+
+BASE = prima.BnbParams(lp=mf.LP)
+FRONTIERS = [1, 8]
+
+
+def olp(p):
+    return orc.OracleLp.from_csr(p.n, p.m, p.n_eq, p.row_ptr, p.col, p.val, p.c, p.q, p.l, p.u,
+                                 p.offset)
+
+
+@pytest.mark.parametrize("frontier", FRONTIERS)
+def test_integer_optimum_differs_from_relaxation(hip64, frontier):
+    p = mf.knapsackish()
+    rel = hip64.solve(p, mf.LP)
+    assert rel.status == "Optimal" and abs(rel.objective_value + 21.0) < 1e-6
+    assert abs(rel.primal[1] - 1.5) < 1e-5
+    r = hip64.milp_solve(p, {0, 1}, BASE.copy(frontier=frontier))
+    assert r.status == "Optimal", r
+    assert abs(r.objective_value + 20.0) < 1e-6
+    assert r.primal[0] == 4.0 and r.primal[1] == 0.0
+    assert all(v == round(v) for v in r.primal)
+    assert r.best_bound <= r.objective_value + 1e-9 and r.gap <= 1e-6
+    assert abs(r.objective_value - (p.offset + float(np.dot(p.c, r.primal)))) <= 1e-12
+
+
+@pytest.mark.parametrize("frontier", FRONTIERS)
+def test_one_integer_column(hip64, frontier):
+    r = hip64.milp_solve(mf.knapsackish(), {0}, BASE.copy(frontier=frontier))
+    assert r.status == "Optimal" and abs(r.objective_value + 21.0) < 1e-6
+    assert r.primal[0] == 3.0 and abs(r.primal[1] - 1.5) < 1e-5
+
+
+def test_empty_integer_set_short_circuits(hip64):
+    r = hip64.milp_solve(mf.knapsackish(), set(), BASE)
+    assert r.status == "Optimal" and abs(r.objective_value + 21.0) < 1e-6
+    assert r.nodes_explored == 1
+
+
+@pytest.mark.parametrize("frontier", FRONTIERS)
+def test_no_integer_point(hip64, frontier):
+    p = mf.half_integer()
+    assert hip64.solve(p, mf.LP).status == "Optimal"
+    r = hip64.milp_solve(p, {0}, BASE.copy(frontier=frontier))
+    assert r.status == "Infeasible", r
+    assert len(r.primal) == 0
+
+
+@pytest.mark.parametrize("frontier", FRONTIERS)
+def test_binaries(hip64, frontier):
+    p = mf.best_two_of_three()
+    r = hip64.milp_solve(p, {0, 1, 2}, BASE.copy(frontier=frontier))
+    assert r.status == "Optimal" and abs(r.objective_value + 7.0) < 1e-6
+    assert list(r.primal) == [1.0, 0.0, 1.0]
+
+
+def test_node_limit_downgrades(hip64):
+    r = hip64.milp_solve(mf.knapsackish(), {0, 1}, BASE.copy(max_nodes=1))
+    assert r.status != "Optimal" and not r.is_conclusive
+
+
+def test_warm_start_does_not_change_the_answer(hip64):
+    cold = hip64.milp_solve(mf.knapsackish(), {0, 1}, BASE.copy(warm_start=False))
+    warm = hip64.milp_solve(mf.knapsackish(), {0, 1}, BASE.copy(warm_start=True))
+    assert warm.status == cold.status
+    assert abs(warm.objective_value - cold.objective_value) <= 1e-9
+    assert list(warm.primal) == list(cold.primal)
+
+
+def test_out_of_range_column_rejected(hip64):
+    with pytest.raises(ValueError):
+        hip64.milp_solve(mf.knapsackish(), {0, 7}, BASE)
+    with pytest.raises(ValueError):
+        hip64.milp_solve(mf.knapsackish(), {0}, BASE.copy(frontier=0))
+
+
+@pytest.mark.parametrize("frontier", [1, 4, 32])
+def test_margin_costs_nodes(hip64, frontier):
+    p = mf.knapsack16()
+    ints = set(range(16))
+    best, _ = mf.brute_force(p, sorted(ints))
+    runs = {f: hip64.milp_solve(p, ints, BASE.copy(pruning_safety_factor=f, frontier=frontier))
+            for f in (0.0, 10.0, 1e9)}
+    for r in runs.values():
+        assert r.status == "Optimal" and abs(r.objective_value - best) <= 1e-6, r
+    assert abs(runs[10.0].objective_value - runs[0.0].objective_value) <= 1e-9
+    assert runs[10.0].nodes_explored >= runs[0.0].nodes_explored
+    assert runs[1e9].nodes_explored > runs[10.0].nodes_explored
+    if frontier > 1:
+        # batched: far fewer device calls than nodes
+        assert runs[10.0].batches < runs[10.0].nodes_explored
+
+
+def test_sequential_search_follows_the_oracle(hip64):
+    # frontier 1 walks the same tree as the oracle's sequential search: same
+    # optimum, same incumbent, and node counts within a few nodes (a node's
+    # relaxation may end a checkpoint apart, which moves a fractional value
+    # by ~1e-9 and can flip a tie in the most-fractional choice)
+    p = mf.knapsack16()
+    ints = list(range(16))
+    o = orc.bnb_solve(olp(p), ints, orc.default_params(eps_abs=1e-9, eps_rel=1e-9))
+    g = hip64.milp_solve(p, set(ints), BASE)
+    assert g.status == o.status == "Optimal"
+    assert abs(g.objective_value - o.objective) <= 1e-9
+    assert list(g.primal) == list(o.primal)
+    assert abs(g.nodes_explored - o.nodes_explored) <= max(2, o.nodes_explored // 20), (g, o)
+
+
+def test_starved_relaxations_are_counted(hip64):
+    starved = BASE.copy(lp=prima.PdhgParams(eps_abs=1e-14, eps_rel=1e-14, max_iterations=2))
+    r = hip64.milp_solve(mf.knapsackish(), {0, 1}, starved)
+    assert r.status != "Optimal" and r.unproven_nodes > 0 and not r.is_conclusive
+
+
+def test_unbounded(hip64):
+    assert hip64.milp_solve(mf.unbounded_integer(), {0}, BASE).status == "Unbounded"
+
+
+def test_time_limit(hip64):
+    p = mf.knapsack16()
+    ints = set(range(16))
+    full = hip64.milp_solve(p, ints, BASE)
+    assert full.status == "Optimal" and full.nodes_explored > 50
+    limited = hip64.milp_solve(p, ints, BASE.copy(time_limit_ms=0))
+    assert limited.nodes_explored <= full.nodes_explored
+    if full.solve_time_ms > 5:
+        assert limited.status != "Optimal"
+        assert limited.nodes_explored < full.nodes_explored
+
+
+def test_no_answer_means_no_point(hip64):
+    r = hip64.milp_solve(mf.outside_the_box(), set(), BASE)
+    assert r.status == "Infeasible" and len(r.primal) == 0
