@@ -22,8 +22,8 @@
  *     iteration, KKT checkpoints, restarts and certificates run on the GPU.
  *
  * Threading: a context is not thread-safe and is used from one thread
- * (Kernels.scala:39-40); prima_cancel is the one call that may come from
- * another thread.
+ * (Kernels.scala:39-40); prima_cancel and prima_progress are the calls that
+ * may come from another thread.
  */
 #ifndef PRIMA_HIP_H
 #define PRIMA_HIP_H
@@ -341,6 +341,14 @@ int prima_stats_reset(prima_ctx* ctx);
  * a running solve on ctx stops at its next launch boundary with status
  * PRIMA_INTERRUPTED. Safe to call from another thread. */
 int prima_cancel(prima_ctx* ctx);
+
+/* Progress of the solves on ctx since it was opened, updated at every launch
+ * boundary: solver launches, accepted PDHG iterations summed over LPs, and
+ * LPs finished. No counterpart in the reference (its solves report nothing
+ * until they return); for callers that watch a long batch. Safe to call from
+ * another thread while a solve runs. */
+int prima_progress(const prima_ctx* ctx, int64_t* launches, int64_t* accepted_iterations,
+                   int64_t* finished_lps);
 
 #ifdef __cplusplus
 }

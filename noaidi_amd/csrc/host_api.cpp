@@ -73,6 +73,9 @@ struct prima_ctx {
   int occ[2] = {0, 0};  // blocks per CU for float / double kernels
   double* scratch = nullptr;  // reduction partials for dot / squaredNorm
   std::atomic<int> cancel{0};
+  // progress of the running solve, readable from another thread
+  std::atomic<int64_t> prog_launches{0}, prog_iterations{0}, prog_finished{0};
+  int64_t prog_base = 0;  // accepted iterations of finished waves
   std::string device_name;
   bool coop_ok = false;
   prima_stats stats{};
@@ -552,7 +555,7 @@ LpLayout carve(Carver& cv, const HostLp& h, int prec, bool with_ylo, const LpLay
   L.y_out = cv.take(m * 8);
   L.part = cv.take((2 * G * kIterParts + G * kEvalParts) * 8);
   L.bar = cv.take(16);
-  L.st = cv.take(sizeof(SolveState));
+  L.st = 0;  // placed by the wave: all states of a wave are contiguous
   L.ylo = with_ylo ? cv.take(m * s) : 0;
   L.oylo = with_ylo ? cv.take(m * 8) : 0;
   return L;
@@ -615,6 +618,10 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
     W.lay.push_back(carve(cv, h, prec, jobs[idx[k]].ylo != nullptr,
                           owner_of[k] >= 0 ? &W.lay[owner_of[k]] : nullptr));
   }
+  // one contiguous block of per-LP states, read back with a single copy at
+  // every launch boundary
+  const size_t st_base = cv.take(idx.size() * sizeof(SolveState));
+  for (size_t k = 0; k < idx.size(); ++k) W.lay[k].st = st_base + k * sizeof(SolveState);
   const size_t arena_bytes = cv.off;
   int grid = 0;
   for (int j : idx) grid += jobs[j].h.G;
@@ -878,10 +885,20 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
       ctx->stats.kernel_ms += ev_ms;
       ctx->stats.launches += 1;
     }
-    for (size_t k = 0; k < nlp; ++k) {
-      HIP_TRY(ctx, hipMemcpy(&states[k], W.arena + W.lay[k].st, sizeof(SolveState),
+    {
+      int64_t wave_iters = 0, newly = 0;
+      HIP_TRY(ctx, hipMemcpy(states.data(), W.arena + st_base, nlp * sizeof(SolveState),
                              hipMemcpyDeviceToHost));
-      if (states[k].status >= 0 && done_ms[k] < 0) done_ms[k] = ms_since(t_solve);
+      for (size_t k = 0; k < nlp; ++k) {
+        if (states[k].status >= 0 && done_ms[k] < 0) {
+          done_ms[k] = ms_since(t_solve);
+          ++newly;
+        }
+        wave_iters += states[k].iteration;
+      }
+      ctx->prog_launches.fetch_add(1);
+      ctx->prog_finished.fetch_add(newly);
+      ctx->prog_iterations.store(ctx->prog_base + wave_iters);
     }
     bool all_done = true;
     for (auto& st : states) all_done &= st.status >= 0;
@@ -904,9 +921,8 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
         ctx->stats.kernel_ms += ev_ms;
         ctx->stats.launches += 1;
       }
-      for (size_t k = 0; k < nlp; ++k)
-        HIP_TRY(ctx, hipMemcpy(&states[k], W.arena + W.lay[k].st, sizeof(SolveState),
-                               hipMemcpyDeviceToHost));
+      HIP_TRY(ctx, hipMemcpy(states.data(), W.arena + st_base, nlp * sizeof(SolveState),
+                             hipMemcpyDeviceToHost));
       break;
     }
     {
@@ -923,6 +939,16 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
     }
   }
   const double solve_ms = ms_since(t_solve);
+  {
+    int64_t wave_iters = 0, newly = 0;
+    for (size_t k = 0; k < nlp; ++k) {
+      wave_iters += states[k].iteration;
+      newly += done_ms[k] < 0 ? 1 : 0;
+    }
+    ctx->prog_base += wave_iters;
+    ctx->prog_iterations.store(ctx->prog_base);
+    ctx->prog_finished.fetch_add(newly);
+  }
   for (size_t k = 0; k < nlp; ++k) {
     SolveJob& J = jobs[idx[k]];
     const LpLayout& L = W.lay[k];
@@ -1841,6 +1867,15 @@ int prima_stats_get(const prima_ctx* ctx, prima_stats* out) {
 int prima_stats_reset(prima_ctx* ctx) {
   if (!ctx) return PRIMA_EINVAL;
   std::memset(&ctx->stats, 0, sizeof(ctx->stats));
+  return PRIMA_OK;
+}
+
+int prima_progress(const prima_ctx* ctx, int64_t* launches, int64_t* accepted_iterations,
+                   int64_t* finished_lps) {
+  if (!ctx) return PRIMA_EINVAL;
+  if (launches) *launches = ctx->prog_launches.load();
+  if (accepted_iterations) *accepted_iterations = ctx->prog_iterations.load();
+  if (finished_lps) *finished_lps = ctx->prog_finished.load();
   return PRIMA_OK;
 }
 

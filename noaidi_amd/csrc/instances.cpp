@@ -27,6 +27,9 @@ typedef struct {
   double known_objective;  // NaN when no closed form exists
   double* x_star;          // known primal optimum (or NULL)
   double* y_star;          // known dual optimum (or NULL)
+  int32_t n_int;           // integer columns (0 for a pure LP)
+  int32_t pad2;
+  int32_t* integers;       // ascending column indices (or NULL)
 } prima_instance;
 }
 
@@ -65,6 +68,41 @@ struct Trip {
   int32_t r, c;
   double v;
 };
+
+// Ring of N buses plus `chords` distinct random chords; reactance U[0.01,0.1],
+// capacity U[100,1000] per line, drawn in that order.
+struct Grid {
+  std::vector<std::pair<int32_t, int32_t>> lines;
+  std::vector<double> x, cap;
+};
+
+Grid make_grid(Rng& rng, int32_t N, int32_t chords) {
+  Grid g;
+  for (int32_t b = 0; b < N; ++b) g.lines.push_back({b, (b + 1) % N});
+  std::vector<uint64_t> keys;
+  for (auto& e : g.lines)
+    keys.push_back((uint64_t)std::min(e.first, e.second) * N + std::max(e.first, e.second));
+  std::sort(keys.begin(), keys.end());
+  int32_t added = 0, guard = 0;
+  while (added < chords && guard < chords * 100) {
+    ++guard;
+    int32_t a = (int32_t)rng.below(N), b = (int32_t)rng.below(N);
+    if (a == b) continue;
+    const uint64_t k = (uint64_t)std::min(a, b) * N + std::max(a, b);
+    if (std::binary_search(keys.begin(), keys.end(), k)) continue;
+    keys.insert(std::lower_bound(keys.begin(), keys.end(), k), k);
+    g.lines.push_back({a, b});
+    ++added;
+  }
+  const size_t L = g.lines.size();
+  g.x.resize(L);
+  g.cap.resize(L);
+  for (size_t l = 0; l < L; ++l) {
+    g.x[l] = rng.uniform(0.01, 0.1);
+    g.cap[l] = rng.uniform(100.0, 1000.0);
+  }
+  return g;
+}
 
 prima_instance* alloc_instance(int32_t n, int32_t m, int64_t nnz) {
   prima_instance* I = (prima_instance*)std::calloc(1, sizeof(prima_instance));
@@ -122,6 +160,7 @@ void prima_instance_free(prima_instance* I) {
   std::free(I->u);
   std::free(I->x_star);
   std::free(I->y_star);
+  std::free(I->integers);
   std::free(I);
 }
 
@@ -226,31 +265,11 @@ prima_instance* prima_gen_dispatch(uint64_t seed, int32_t buses, int32_t chords,
   if (buses < 3 || snapshots < 1 || gens_per_bus < 1 || storage < 0) return nullptr;
   Rng rng(seed);
   const int32_t N = buses, T = snapshots, S = storage;
-  std::vector<std::pair<int32_t, int32_t>> lines;
-  std::vector<std::vector<uint8_t>> adj;  // only for small N checks
-  for (int32_t b = 0; b < N; ++b) lines.push_back({b, (b + 1) % N});
-  {
-    std::vector<uint64_t> keys;
-    for (auto& e : lines) keys.push_back((uint64_t)std::min(e.first, e.second) * N + std::max(e.first, e.second));
-    std::sort(keys.begin(), keys.end());
-    int32_t added = 0, guard = 0;
-    while (added < chords && guard < chords * 100) {
-      ++guard;
-      int32_t a = (int32_t)rng.below(N), b = (int32_t)rng.below(N);
-      if (a == b) continue;
-      const uint64_t k = (uint64_t)std::min(a, b) * N + std::max(a, b);
-      if (std::binary_search(keys.begin(), keys.end(), k)) continue;
-      keys.insert(std::lower_bound(keys.begin(), keys.end(), k), k);
-      lines.push_back({a, b});
-      ++added;
-    }
-  }
+  const Grid grid = make_grid(rng, N, chords);
+  const auto& lines = grid.lines;
+  const auto& xl = grid.x;
+  const auto& cap = grid.cap;
   const int32_t L = (int32_t)lines.size();
-  std::vector<double> xl(L), cap(L);
-  for (int32_t l = 0; l < L; ++l) {
-    xl[l] = rng.uniform(0.01, 0.1);
-    cap[l] = rng.uniform(100.0, 1000.0);
-  }
   const int32_t G = N * gens_per_bus;
   std::vector<double> gcost(G), gnom(G);
   for (int32_t g = 0; g < G; ++g) {
@@ -356,6 +375,125 @@ prima_instance* prima_gen_dispatch(uint64_t seed, int32_t buses, int32_t chords,
       I->u[vsoc(t, k)] = 4.0 * snom[k];
     }
   }
+  return I;
+}
+
+// Multi-bus unit commitment over the dispatch network (DC flows in angle
+// form, load shedding at 3000/MWh), with a binary on/off variable per
+// generator and snapshot. Per snapshot t and generator g: output p, status u
+// (integer, [0,1]), start-up su and shut-down sd in [0,1]. Rows, equalities
+// first: nodal balance and flow definition per snapshot; then
+//   p - pmin*u >= 0,  pnom*u - p >= 0,
+//   su[t] - u[t] + u[t-1] >= 0,  sd[t] - u[t-1] + u[t] >= 0 (t > 0),
+//   u[t] - sum(su[t-min_up+1 .. t]) >= 0                     (minimum up time)
+//   -u[t] - sum(sd[t-min_down+1 .. t]) >= -1                 (minimum down time)
+// with every unit off before t = 0 (sd[0] fixed at 0). Costs: marginal
+// U[10,100], start-up U[100,1000]; p_nom U[50,500], p_min/p_nom U[0.2,0.5].
+prima_instance* prima_gen_uc(uint64_t seed, int32_t buses, int32_t chords, int32_t snapshots,
+                             int32_t gens_per_bus, int32_t min_up, int32_t min_down) {
+  if (buses < 3 || snapshots < 1 || gens_per_bus < 1 || min_up < 1 || min_down < 1)
+    return nullptr;
+  Rng rng(seed);
+  const int32_t N = buses, T = snapshots;
+  const Grid grid = make_grid(rng, N, chords);
+  const int32_t L = (int32_t)grid.lines.size();
+  const int32_t G = N * gens_per_bus;
+  std::vector<double> gcost(G), gnom(G), gmin(G), gstart(G);
+  for (int32_t g = 0; g < G; ++g) {
+    gcost[g] = rng.uniform(10.0, 100.0);
+    gnom[g] = rng.uniform(50.0, 500.0);
+    gmin[g] = rng.uniform(0.2, 0.5) * gnom[g];
+    gstart[g] = rng.uniform(100.0, 1000.0);
+  }
+  std::vector<double> base(N), noise(N);
+  for (int32_t b = 0; b < N; ++b) {
+    base[b] = rng.uniform(100.0, 400.0);
+    noise[b] = rng.uniform(0.8, 1.2);
+  }
+  auto load = [&](int32_t b, int32_t t) {
+    const double hour = (double)(t % 24);
+    return base[b] * (1.0 + 0.25 * std::sin(2.0 * M_PI * (hour - 8.0) / 24.0)) * noise[b];
+  };
+  const int32_t per_t = 4 * G + L + 2 * N;
+  const int32_t n = per_t * T;
+  auto vp = [&](int32_t t, int32_t g) { return t * per_t + g; };
+  auto vu = [&](int32_t t, int32_t g) { return t * per_t + G + g; };
+  auto vsu = [&](int32_t t, int32_t g) { return t * per_t + 2 * G + g; };
+  auto vsd = [&](int32_t t, int32_t g) { return t * per_t + 3 * G + g; };
+  auto vf = [&](int32_t t, int32_t l) { return t * per_t + 4 * G + l; };
+  auto vth = [&](int32_t t, int32_t b) { return t * per_t + 4 * G + L + b; };
+  auto vshed = [&](int32_t t, int32_t b) { return t * per_t + 4 * G + L + N + b; };
+  const int32_t m_eq = (N + L) * T;
+  const int32_t ineq_per_t = 6 * G;
+  const int32_t m = m_eq + ineq_per_t * T;
+  std::vector<Trip> tr;
+  std::vector<double> q((size_t)m, 0.0);
+  for (int32_t t = 0; t < T; ++t) {
+    const int32_t rb = t * N, rf = T * N + t * L;
+    for (int32_t g = 0; g < G; ++g) tr.push_back({rb + g / gens_per_bus, vp(t, g), 1.0});
+    for (int32_t l = 0; l < L; ++l) {
+      const int32_t a = grid.lines[l].first, b = grid.lines[l].second;
+      tr.push_back({rb + b, vf(t, l), 1.0});
+      tr.push_back({rb + a, vf(t, l), -1.0});
+      tr.push_back({rf + l, vf(t, l), 1.0});
+      tr.push_back({rf + l, vth(t, a), -1.0 / grid.x[l]});
+      tr.push_back({rf + l, vth(t, b), 1.0 / grid.x[l]});
+    }
+    for (int32_t b = 0; b < N; ++b) {
+      tr.push_back({rb + b, vshed(t, b), 1.0});
+      q[rb + b] = load(b, t);
+    }
+    const int32_t ri = m_eq + t * ineq_per_t;
+    for (int32_t g = 0; g < G; ++g) {
+      const int32_t r0 = ri + 6 * g;
+      tr.push_back({r0, vp(t, g), 1.0});  // p - pmin u >= 0
+      tr.push_back({r0, vu(t, g), -gmin[g]});
+      tr.push_back({r0 + 1, vu(t, g), gnom[g]});  // pnom u - p >= 0
+      tr.push_back({r0 + 1, vp(t, g), -1.0});
+      tr.push_back({r0 + 2, vsu(t, g), 1.0});  // su - u[t] + u[t-1] >= 0
+      tr.push_back({r0 + 2, vu(t, g), -1.0});
+      if (t > 0) tr.push_back({r0 + 2, vu(t - 1, g), 1.0});
+      tr.push_back({r0 + 3, vsd(t, g), 1.0});  // sd - u[t-1] + u[t] >= 0
+      tr.push_back({r0 + 3, vu(t, g), 1.0});
+      if (t > 0) tr.push_back({r0 + 3, vu(t - 1, g), -1.0});
+      tr.push_back({r0 + 4, vu(t, g), 1.0});  // u[t] - sum su >= 0
+      for (int32_t k = std::max(0, t - min_up + 1); k <= t; ++k)
+        tr.push_back({r0 + 4, vsu(k, g), -1.0});
+      tr.push_back({r0 + 5, vu(t, g), -1.0});  // -u[t] - sum sd >= -1
+      for (int32_t k = std::max(0, t - min_down + 1); k <= t; ++k)
+        tr.push_back({r0 + 5, vsd(k, g), -1.0});
+      q[r0 + 5] = -1.0;
+    }
+  }
+  prima_instance* I = from_triplets(n, m, tr);
+  I->n_eq = m_eq;
+  std::memcpy(I->q, q.data(), (size_t)m * sizeof(double));
+  const double inf = std::numeric_limits<double>::infinity();
+  I->n_int = G * T;
+  I->integers = (int32_t*)std::malloc((size_t)I->n_int * sizeof(int32_t));
+  int32_t k = 0;
+  for (int32_t t = 0; t < T; ++t) {
+    for (int32_t g = 0; g < G; ++g) {
+      I->c[vp(t, g)] = gcost[g];
+      I->u[vp(t, g)] = gnom[g];
+      I->u[vu(t, g)] = 1.0;
+      I->c[vsu(t, g)] = gstart[g];
+      I->u[vsu(t, g)] = 1.0;
+      I->u[vsd(t, g)] = t == 0 ? 0.0 : 1.0;
+      I->integers[k++] = vu(t, g);
+    }
+    for (int32_t l = 0; l < L; ++l) {
+      I->l[vf(t, l)] = -grid.cap[l];
+      I->u[vf(t, l)] = grid.cap[l];
+    }
+    for (int32_t b = 0; b < N; ++b) {
+      I->l[vth(t, b)] = b == 0 ? 0.0 : -inf;
+      I->u[vth(t, b)] = b == 0 ? 0.0 : inf;
+      I->c[vshed(t, b)] = 3000.0;
+      I->u[vshed(t, b)] = load(b, t);
+    }
+  }
+  std::sort(I->integers, I->integers + I->n_int);
   return I;
 }
 

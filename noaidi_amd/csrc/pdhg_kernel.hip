@@ -221,21 +221,41 @@ struct EpiDual {  // rows of A: kxBar and the projected dual step (ScalaKernels.
   T sigma;
   int n_eq;
   T dy2, dykdx;
+  // The row-side commit of the previous accepted iteration (Pdhg.scala:373-377,
+  // 384-389), deferred into this pass: kx = (kxBar + kx)/2, y = yNext and the
+  // running sums, applied before the dual step reads y and kx.
+  bool commit;
+  T w;
   struct Ops {
-    T y, q, kx, lo;
+    T y, q, kx, lo, kb, sy, skx;
   };
   __device__ Ops load(int r) const {
     Ops o;
-    o.y = L.y()[r];
+    if (commit) {
+      o.y = L.yn()[r];
+      o.kb = L.kxbar()[r];
+      o.sy = L.sy()[r];
+      o.skx = L.skx()[r];
+    } else {
+      o.y = L.y()[r];
+    }
     o.q = L.q()[r];
     o.kx = L.kx()[r];
     o.lo = (r >= n_eq && L.v.ylo) ? ((const T*)L.v.ylo)[r] : T(0);
     return o;
   }
   __device__ void apply(int r, T s, const Ops& o) {
+    T kxv = o.kx;
+    if (commit) {
+      kxv = T(0.5) * o.kb + T(0.5) * o.kx;
+      L.kx()[r] = kxv;
+      L.y()[r] = o.y;
+      L.sy()[r] = w * o.y + o.sy;
+      L.skx()[r] = w * kxv + o.skx;
+    }
     T ynew = o.y + sigma * (o.q - s);
     if (r >= n_eq && !(ynew > o.lo)) ynew = o.lo;
-    const T kdx = T(0.5) * s - T(0.5) * o.kx;
+    const T kdx = T(0.5) * s - T(0.5) * kxv;
     const T d = ynew - o.y;
     L.yn()[r] = ynew;
     L.kxbar()[r] = s;
@@ -576,6 +596,22 @@ __device__ __noinline__ void commit_columns_cold(const LpView& v, const Ranges& 
   (void)commit_columns<T>(v, R, w, false, T(0), sm);
 }
 
+// The deferred row-side commit on its own, before a checkpoint or the final
+// evaluation: the same arithmetic as EpiDual's commit branch.
+template <typename T>
+__device__ __noinline__ void commit_rows_cold(const LpView& v, const Ranges& R, T w) {
+  const Lp<T> L(v);
+  for (int r = R.r0 + threadIdx.x; r < R.r1; r += NT) {
+    const T kxv = T(0.5) * L.kxbar()[r] + T(0.5) * L.kx()[r];
+    const T yv = L.yn()[r];
+    L.kx()[r] = kxv;
+    L.y()[r] = yv;
+    L.sy()[r] = w * yv + L.sy()[r];
+    L.skx()[r] = w * kxv + L.skx()[r];
+  }
+  __syncthreads();
+}
+
 // Initialisation on the device (Pdhg.scala:305-309): kty = A^T y, restart
 // point, zero running sums.
 template <typename T>
@@ -708,7 +744,8 @@ __global__ __launch_bounds__(NT) void pdhg_persistent(const LpView* __restrict__
   }
 
   const int it_start = S.iteration;
-  bool pending = false;
+  bool pending = false;       // column-side commit owed (applied in phase 1)
+  bool pending_rows = false;  // row-side commit owed (applied in phase 2)
   T wpend = T(0);
 
   for (;;) {
@@ -735,9 +772,10 @@ __global__ __launch_bounds__(NT) void pdhg_persistent(const LpView* __restrict__
 
     // phase 2: own rows
     {
-      EpiDual<T> epi{L, sigma, v.n_eq, T(0), T(0)};
+      EpiDual<T> epi{L, sigma, v.n_eq, T(0), T(0), pending_rows, wpend};
       spmv_blocks(v.a_ptr, v.a_col, L.aval(), (const T*)L.xbar(), v.a_rb, R.ab0, R.ab1, tile,
                   reinterpret_cast<T*>(sm.rsum), sm.rptr, epi);
+      pending_rows = false;
       double a[2] = {(double)epi.dy2, (double)epi.dykdx};
       block_sum<2>(a, sm.red);
       if (threadIdx.x == 0) {
@@ -774,18 +812,10 @@ __global__ __launch_bounds__(NT) void pdhg_persistent(const LpView* __restrict__
     const bool accepted = S.step <= limit || S.trials >= 60;
 
     if (accepted) {
-      const T w = (T)S.step;
-      for (int r = R.r0 + threadIdx.x; r < R.r1; r += NT) {
-        const T kxv = T(0.5) * L.kxbar()[r] + T(0.5) * L.kx()[r];
-        const T yv = L.yn()[r];
-        L.kx()[r] = kxv;
-        L.y()[r] = yv;
-        L.sy()[r] = w * yv + L.sy()[r];
-        L.skx()[r] = w * kxv + L.skx()[r];
-      }
       S.wsum += S.step;
       pending = true;
-      wpend = w;
+      pending_rows = true;
+      wpend = (T)S.step;
       S.iteration++;
       S.trials = 0;
     }
@@ -795,6 +825,8 @@ __global__ __launch_bounds__(NT) void pdhg_persistent(const LpView* __restrict__
     bool evaluated_now = false;
     if (S.iteration % P.eval_freq == 0) {
       // checkpoint (Pdhg.scala:396-417)
+      commit_rows_cold<T>(v, R, wpend);
+      pending_rows = false;
       commit_columns_cold<T>(v, R, wpend, sm);
       pending = false;
       if (!group_barrier(v.bar, G)) return abort_launch(v, R, S);
@@ -809,6 +841,10 @@ __global__ __launch_bounds__(NT) void pdhg_persistent(const LpView* __restrict__
     }
 
     if (S.iteration >= P.max_iter) {
+      if (pending_rows) {
+        commit_rows_cold<T>(v, R, wpend);
+        pending_rows = false;
+      }
       if (pending) {
         commit_columns_cold<T>(v, R, wpend, sm);
         pending = false;

@@ -142,7 +142,9 @@ class _Instance(C.Structure):
                 ("c", C.POINTER(C.c_double)), ("q", C.POINTER(C.c_double)),
                 ("l", C.POINTER(C.c_double)), ("u", C.POINTER(C.c_double)),
                 ("offset", C.c_double), ("known_objective", C.c_double),
-                ("x_star", C.POINTER(C.c_double)), ("y_star", C.POINTER(C.c_double))]
+                ("x_star", C.POINTER(C.c_double)), ("y_star", C.POINTER(C.c_double)),
+                ("n_int", C.c_int32), ("pad2", C.c_int32),
+                ("integers", C.POINTER(C.c_int32))]
 
 
 # Every symbol include/prima_hip.h declares.
@@ -155,7 +157,7 @@ EXPORTED = [
     "prima_mixed_solve", "prima_solve_batch", "prima_cancel", "prima_stats_get",
     "prima_stats_reset", "prima_mat_set_schedule", "prima_mat_autotune",
     "prima_default_bnb_params", "prima_milp_solve", "prima_bnb_skips_before_solving",
-    "prima_bnb_safe_bound_of",
+    "prima_bnb_safe_bound_of", "prima_progress",
 ]
 
 _lib = None
@@ -213,6 +215,7 @@ def lib():
                                        C.POINTER(_Mixed), C.POINTER(_MilpResult), vp]),
             "prima_bnb_skips_before_solving": (i32, [d, d, i32]),
             "prima_bnb_safe_bound_of": (d, [i32, d, d]),
+            "prima_progress": (i32, [vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(i64)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -234,6 +237,9 @@ def instances_lib():
         L.prima_gen_dispatch.restype = C.POINTER(_Instance)
         L.prima_gen_dispatch.argtypes = [C.c_uint64, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                          C.c_int32, C.c_double, C.c_double, C.c_uint64]
+        L.prima_gen_uc.restype = C.POINTER(_Instance)
+        L.prima_gen_uc.argtypes = [C.c_uint64, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                   C.c_int32, C.c_int32]
         L.prima_instance_free.argtypes = [C.POINTER(_Instance)]
         _ilib = L
     return _ilib
@@ -265,6 +271,7 @@ class LpProblem:
     known_objective: float = float("nan")
     x_star: Optional[np.ndarray] = None
     y_star: Optional[np.ndarray] = None
+    integers: Optional[np.ndarray] = None  # integer columns of a generated MILP
 
     def __post_init__(self):
         self.row_ptr = np.ascontiguousarray(self.row_ptr, dtype=np.int32)
@@ -782,6 +789,13 @@ class HipKernels:
         self._check(lib().prima_stats_get(self._h, C.byref(st)), "stats")
         return {f[0]: getattr(st, f[0]) for f in _Stats._fields_}
 
+    def progress(self) -> dict:
+        """Launches, accepted iterations and finished LPs so far; may be called
+        from another thread while a solve runs on this context."""
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        self._check(lib().prima_progress(self._h, C.byref(a), C.byref(b), C.byref(c)), "progress")
+        return dict(launches=a.value, accepted_iterations=b.value, finished_lps=c.value)
+
     def reset_stats(self) -> None:
         self._check(lib().prima_stats_reset(self._h), "reset_stats")
 
@@ -966,6 +980,8 @@ def _from_instance(ptr) -> LpProblem:
         prob.x_star = arr(I.x_star, n, np.float64)
     if I.y_star:
         prob.y_star = arr(I.y_star, m, np.float64)
+    if I.integers:
+        prob.integers = arr(I.integers, I.n_int, np.int32)
     instances_lib().prima_instance_free(ptr)
     return prob
 
@@ -985,6 +1001,15 @@ def dispatch(seed: int, buses: int = 200, chords: int = 100, snapshots: int = 24
     return _from_instance(instances_lib().prima_gen_dispatch(seed, buses, chords, snapshots,
                                                              gens_per_bus, storage, efficiency,
                                                              perturb, scenario_seed))
+
+
+def unit_commitment(seed: int, buses: int = 100, chords: int = 50, snapshots: int = 24,
+                    gens_per_bus: int = 3, min_up: int = 3, min_down: int = 3) -> LpProblem:
+    """Multi-bus unit commitment over the dispatch network: binary on/off per
+    generator and snapshot (`.integers`), minimum up/down times, start-up
+    costs U[100,1000], DC flows. The LP is its relaxation."""
+    return _from_instance(instances_lib().prima_gen_uc(seed, buses, chords, snapshots,
+                                                       gens_per_bus, min_up, min_down))
 
 
 def config0(seed: int = 0) -> LpProblem:

@@ -88,3 +88,34 @@ def brute_force(p: prima.LpProblem, integers, box=None):
     obj = X[ok] @ p.c + p.offset
     k = int(np.argmin(obj))
     return float(obj[k]), X[ok][k]
+
+
+GOLDEN_DIR = __import__("os").path.join(__import__("os").path.dirname(__file__), "golden")
+
+
+def uc_milp_goldens():
+    """Small unit-commitment MILPs with their HiGHS MILP optimum."""
+    import json
+    import os
+    with open(os.path.join(GOLDEN_DIR, "highs_uc_milp.json")) as f:
+        return json.load(f)
+
+
+def uc_frontier():
+    """The 100-bus, 24-snapshot UC relaxation and its depth-9 frontier: node k
+    fixes branch column i to bit i of k. Returns (root, golden, node(k))."""
+    import json
+    import os
+    with open(os.path.join(GOLDEN_DIR, "highs_uc.json")) as f:
+        g = json.load(f)
+    root = prima.unit_commitment(**g["args"])
+    cols = g["branch_columns"]
+
+    def node(k: int) -> prima.LpProblem:
+        l, u = root.l.copy(), root.u.copy()
+        for i, c in enumerate(cols):
+            l[c] = u[c] = float((k >> i) & 1)
+        return prima.LpProblem(root.n, root.m, root.n_eq, root.row_ptr, root.col, root.val,
+                               root.c, root.q, l, u, root.offset)
+
+    return root, g, node

@@ -145,3 +145,73 @@ def test_time_limit(hip64):
 def test_no_answer_means_no_point(hip64):
     r = hip64.milp_solve(mf.outside_the_box(), set(), BASE)
     assert r.status == "Infeasible" and len(r.primal) == 0
+
+
+def _check_point(p, r):
+    # integral where required, c'x at the reported x, and feasible to 1e-6
+    assert all(r.primal[j] == round(r.primal[j]) for j in p.integers)
+    assert abs(r.objective_value - (p.offset + float(np.dot(p.c, r.primal)))) <= 1e-9 * max(
+        1.0, abs(r.objective_value))
+    ax = p.to_scipy() @ r.primal
+    scale = 1.0 + np.abs(p.q)
+    assert np.all(np.abs(ax[: p.n_eq] - p.q[: p.n_eq]) <= 1e-6 * scale[: p.n_eq])
+    assert np.all(ax[p.n_eq:] - p.q[p.n_eq:] >= -1e-6 * scale[p.n_eq:])
+    assert np.all(r.primal >= p.l - 1e-9) and np.all(r.primal <= p.u + 1e-9)
+
+
+@pytest.mark.parametrize("case", range(3))
+@pytest.mark.parametrize("frontier", [16, 64])
+def test_unit_commitment_matches_highs_milp(hip64, case, frontier):
+    # small multi-bus UC MILPs closed by the search: HiGHS's MILP optimum to 1e-8
+    g = mf.uc_milp_goldens()[case]
+    p = prima.unit_commitment(**g["args"])
+    r = hip64.milp_solve(p, p.integers, prima.BnbParams(frontier=frontier))
+    assert r.status == "Optimal", r
+    assert abs(r.objective_value - g["milp_objective"]) / abs(g["milp_objective"]) <= 1e-8
+    assert r.best_bound <= r.objective_value + 1e-9 and r.unproven_nodes == 0
+    _check_point(p, r)
+    if frontier > 1:
+        assert r.batches < r.nodes_explored
+
+
+def test_unit_commitment_sequential_follows_oracle(hip64):
+    # frontier 1 against the oracle's sequential search on the same MILP: same
+    # optimum; the trees differ where a node's relaxation differs in the last
+    # bits and flips a most-fractional choice, so node counts agree only
+    # roughly (measured 119 vs 103)
+    g = mf.uc_milp_goldens()[0]
+    p = prima.unit_commitment(**g["args"])
+    o = orc.bnb_solve(olp(p), list(p.integers), orc.default_params())
+    r = hip64.milp_solve(p, p.integers, prima.BnbParams())
+    assert r.status == o.status == "Optimal"
+    assert abs(r.objective_value - o.objective) / abs(o.objective) <= 1e-9
+    assert abs(r.nodes_explored - o.nodes_explored) <= max(2, o.nodes_explored // 4), (
+        r.nodes_explored, o.nodes_explored)
+
+
+def test_unit_commitment_mixed_precision_nodes(hip32):
+    # node relaxations as float32 passes finished in float64, same optimum
+    g = mf.uc_milp_goldens()[0]
+    p = prima.unit_commitment(**g["args"])
+    r = hip32.milp_solve(p, p.integers, prima.BnbParams(frontier=16), mixed=prima.MixedParams())
+    assert r.status == "Optimal", r
+    assert abs(r.objective_value - g["milp_objective"]) / abs(g["milp_objective"]) <= 1e-8
+    _check_point(p, r)
+
+
+def test_uc_frontier_nodes_share_one_matrix(hip64):
+    # the 100-bus, 24-snapshot UC relaxation and a node of its depth-9
+    # frontier (same K, nine binaries fixed by bounds), HiGHS objectives to
+    # 1e-8. The node is a hard LP for restarted PDHG: the oracle needs 546,688
+    # iterations for it from a cold start (the root needs about 119k)
+    root, g, node = mf.uc_frontier()
+    assert (root.n, root.m, root.nnz) == (g["n"], g["m"], g["nnz"])
+    P = prima.PdhgParams(max_iterations=800_000, time_limit_ms=150_000)
+    r = hip64.solve(root, P)
+    assert r.status == "Optimal"
+    assert abs(r.objective_value - g["root_objective"]) / abs(g["root_objective"]) <= 1e-8
+    s = hip64.solve(node(0), P)
+    gold = g["nodes"][0]["objective"]
+    assert s.status == "Optimal", (s.status, s.iterations)
+    assert abs(s.objective_value - gold) / abs(gold) <= 1e-8, (s.objective_value, gold)
+    assert abs(s.iterations - 546_688) <= 0.1 * 546_688, s.iterations

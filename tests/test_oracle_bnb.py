@@ -123,3 +123,15 @@ def test_unbounded_relaxation():
 def test_no_answer_means_no_point():
     r = orc.bnb_solve(olp(mf.outside_the_box()), [], params())
     assert r.status == "Infeasible" and len(r.primal) == 0
+
+
+@pytest.mark.parametrize("case", range(3))
+def test_unit_commitment_matches_highs_milp(case):
+    # independent pin: HiGHS's MILP optimum of small multi-bus UC instances
+    from noaidi_amd import prima
+    g = mf.uc_milp_goldens()[case]
+    p = prima.unit_commitment(**g["args"])
+    r = orc.bnb_solve(olp(p), list(p.integers), orc.default_params(), max_nodes=2000)
+    assert r.status == "Optimal"
+    assert abs(r.objective - g["milp_objective"]) / abs(g["milp_objective"]) <= 1e-8
+    assert all(r.primal[j] in (0.0, 1.0) for j in p.integers)
