@@ -45,6 +45,10 @@ sys.path.insert(0, ROOT)
 from noaidi_amd import prima  # noqa: E402
 
 PEAK_HBM_GBS = 8000.0
+# HBM bytes per algorithmic byte of pdhg_persistent, measured with rocprofv3
+# FETCH_SIZE and WRITE_SIZE (calibrated on a 512 MiB device copy) on 256
+# dispatch LPs: profiles/pdhg_persistent_dispatch200x24_batch256_kernel.md
+TRAFFIC_PER_ALGORITHMIC_BYTE = {64: 0.81, 32: 0.75}
 GOLDEN = os.path.join(ROOT, "tests", "golden", "highs_dispatch.json")
 
 
@@ -363,6 +367,7 @@ def main():
             "n": batch[0].n, "m": batch[0].m, "nnz": batch[0].nnz,
             "tolerance": 1e-8,
         },
+        "device_lps": B * args.steps / (st["kernel_ms"] * 1e-3) if st["kernel_ms"] else None,
         "fp64_unbatched_lps": fp64_lps,
         "fp64_unbatched_sample": U,
         "speedup_vs_fp64_unbatched": value / fp64_lps,
@@ -377,7 +382,10 @@ def main():
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",
             "frac": achieved / PEAK_HBM_GBS,
-            "traffic": None,
+            "traffic": TRAFFIC_PER_ALGORITHMIC_BYTE[64] * st["algorithmic_bytes"] / launches,
+            "traffic_basis": "PMC FETCH_SIZE+WRITE_SIZE per algorithmic byte of the float64 "
+                             "kernel on this workload (0.81; float32 0.75), times this run's "
+                             "algorithmic bytes per launch",
             "algorithmic_bytes_per_launch": st["algorithmic_bytes"] / launches,
             "avg_launch_ms": st["kernel_ms"] / launches,
             "launches": st["launches"],

@@ -119,3 +119,68 @@ def uc_frontier():
                                root.c, root.q, l, u, root.offset)
 
     return root, g, node
+
+
+# ---------------------------------------------------------------------------
+# MilpLadder (MilpLadder.scala:14-131) and its published run
+# (modules/prima-core/NOTES.md:512-522): status Optimal everywhere, objective
+# and B&B node count per instance, at the ladder's params (MilpLadder.scala:93-94).
+
+LADDER_LP = prima.PdhgParams(eps_abs=1e-9, eps_rel=1e-9, max_iterations=200_000)
+LADDER_MAX_NODES = 50_000
+
+
+def _knapsack(values, weights, capacity):
+    b = prima.builder(len(values))
+    for i, v in enumerate(values):
+        b.objectiveCoefficient(i, -float(v))
+        b.bounds(i, 0.0, 1.0)
+    b.lessThan([(i, float(w)) for i, w in enumerate(weights)], float(capacity))
+    return _build(b), list(range(len(values)))
+
+
+def _set_cover(sets, elements):
+    b = prima.builder(len(sets))
+    for j in range(len(sets)):
+        b.objectiveCoefficient(j, 1.0)
+        b.bounds(j, 0.0, 1.0)
+    for e in range(elements):
+        covering = [(j, 1.0) for j, s in enumerate(sets) if e in s]
+        assert covering
+        b.greaterThan(covering, 1.0)
+    return _build(b), list(range(len(sets)))
+
+
+def random_mixed(seed: int, n: int, m: int):
+    """MilpLadder.randomMixed (MilpLadder.scala:71-84), drawn from the
+    java.util.Random restatement in the oracle in the reference's order."""
+    from oracle.oracle import JRandom
+    rng = JRandom(seed)
+    b = prima.builder(n)
+    for j in range(n):
+        b.objectiveCoefficient(j, -(1.0 + rng.next_int(9)))
+        b.bounds(j, 0.0, 1.0 + rng.next_int(3))
+    for _ in range(m):
+        keep = [j for j in range(n) if rng.next_double() < 0.6]
+        terms = [(j, 1.0 + float(rng.next_int(5))) for j in keep]
+        if terms:
+            b.lessThan(terms, 3.0 + rng.next_int(10))
+    return _build(b), [j for j in range(n) if j % 2 == 0]
+
+
+def milp_ladder():
+    """name -> (problem, integer columns, published objective, published nodes)."""
+    k5 = [{0, 1}, {1, 2}, {2, 3}, {3, 4}, {4, 0}, {0, 2}, {1, 3}, {2, 4}, {3, 0}, {4, 1}]
+    out = {
+        "fractional-pair": (knapsackish(), [0, 1], -20.0, 5),
+        "knapsack-8": _knapsack([10, 13, 18, 31, 7, 15, 22, 9], [3, 4, 5, 9, 2, 5, 7, 3], 17) +
+        (-59.0, 23),
+        "knapsack-16": (knapsack16(), list(range(16)), -271.0, 305),
+        "set-cover-triangle": _set_cover([{0, 1}, {1, 2}, {0, 2}], 3) + (2.0, 5),
+        "set-cover-k5-edges": _set_cover(k5, 5) + (3.0, 59),
+        "random-mixed-1": random_mixed(1, 8, 4) + (-27.8, 5),
+        "random-mixed-9": random_mixed(9, 10, 5) + (-31.4, 7),
+        "random-mixed-3": random_mixed(3, 12, 6) + (-14.0, 9),
+        "random-mixed-4": random_mixed(4, 14, 7) + (-33.512820, 5),
+    }
+    return out

@@ -135,3 +135,15 @@ def test_unit_commitment_matches_highs_milp(case):
     assert r.status == "Optimal"
     assert abs(r.objective - g["milp_objective"]) / abs(g["milp_objective"]) <= 1e-8
     assert all(r.primal[j] in (0.0, 1.0) for j in p.integers)
+
+
+@pytest.mark.parametrize("name", list(mf.milp_ladder()))
+def test_milp_ladder_matches_published_run(name):
+    # NOTES.md:512-522 of prima-core: Optimal, objective (6 dp) and node count
+    p, ints, obj, nodes = mf.milp_ladder()[name]
+    r = orc.bnb_solve(olp(p), ints, orc.default_params(eps_abs=1e-9, eps_rel=1e-9,
+                                                       max_iterations=200_000),
+                      max_nodes=mf.LADDER_MAX_NODES)
+    assert r.status == "Optimal" and r.unproven_nodes == 0
+    assert abs(r.objective - obj) <= 1e-6  # published to 6 dp
+    assert r.nodes_explored == nodes
