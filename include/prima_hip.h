@@ -104,7 +104,13 @@ int prima_mat_free(prima_ctx* ctx, prima_mat* m);
 #define PRIMA_SPMV_VECTOR_1 1
 #define PRIMA_SPMV_VECTOR_64 7
 #define PRIMA_SPMV_ROW_BLOCKS 8
-/* Force a schedule for a matrix. */
+/* Column panels: for matrices whose x is larger than an XCD's L2 (at least
+ * 2^20 columns in float32, 2^19 in float64), prima_mat_upload_csr also
+ * stores the entries cut into (row range, column panel) tiles so that the
+ * gathers of x stay in L2. Only available on such matrices. */
+#define PRIMA_SPMV_PANELS 9
+/* Force a schedule for a matrix (PRIMA_EINVAL for an unknown schedule or a
+ * panel schedule on a matrix without panels). */
 int prima_mat_set_schedule(prima_ctx* ctx, prima_mat* a, int32_t schedule);
 /* Autotuner: time every schedule on this matrix (reps launches each, HIP
  * events on the context stream) and keep the fastest for later prima_spmv
@@ -209,7 +215,13 @@ typedef struct {
  * Gleixner, Steffy & Wolter, INFORMS J. Comput. 28(3), 2016): residuals
  * b - Ax and c - A'y and the KKT test in float64, correction LPs on the same
  * device matrix solved by float32 PDHG, at most max_refinements rounds, then
- * the reference's float64 pass if the test still fails. */
+ * the reference's float64 pass if the test still fails.
+ * stall_iterations > 0 (no counterpart in the reference; 0 keeps its
+ * contract) caps the warm-started float64 pass at that many iterations; an LP
+ * still unconverged then gets a cold float64 pass with the rest of
+ * max_iterations. A warm start can put the adaptive primal weight on a path
+ * that stalls for hundreds of thousands of iterations where a cold start
+ * converges. */
 typedef struct {
   double reduced_tolerance;
   int32_t reduced_max_iterations;
@@ -219,7 +231,7 @@ typedef struct {
   int32_t max_refinements;
   double inner_tolerance;
   int32_t inner_max_iterations;
-  int32_t pad;
+  int32_t stall_iterations;
 } prima_mixed_params;
 
 /* MixedPrecision.Result scalars (MixedPrecision.scala:82-91); rounds and

@@ -43,6 +43,13 @@ hipError_t op_spmv_vector(int prec, int tpr, int rows, const int32_t* ptr, const
 hipError_t op_spmv_blocks(int prec, int nblocks, int bpw, const int32_t* ptr, const int32_t* col,
                           const void* val, const int32_t* rbd, const void* x, void* out,
                           hipStream_t s);
+hipError_t op_spmv_panels(int prec, int P, int nranges, const int32_t* range_row0,
+                          const int32_t* tile_off, const uint16_t* starts, const int32_t* pcol,
+                          const void* pval, int nlong, const int32_t* long_rows,
+                          const int32_t* ptr, const int32_t* col, const void* val, const void* x,
+                          void* out, hipStream_t s);
+int panel_rows();
+int panel_tile();
 }  // namespace prima
 
 using namespace prima;
@@ -98,6 +105,14 @@ struct prima_mat {
   int32_t* rbd = nullptr;  // row-block descriptors for the row-block schedule
   int32_t nblocks = 0;
   int32_t schedule = 0;    // PRIMA_SPMV_* (0: by mean row length)
+  // column-panel layout (PRIMA_SPMV_PANELS); panels == 0 when not built
+  int32_t panels = 0, nranges = 0, nlong = 0;
+  int32_t* range_row0 = nullptr;  // nranges + 1 row offsets
+  int32_t* tile_off = nullptr;    // nranges * panels + 1 entry offsets
+  uint16_t* starts = nullptr;     // per tile: row starts relative to the tile
+  int32_t* pcol = nullptr;
+  void* pval = nullptr;
+  int32_t* long_rows = nullptr;   // rows summed outside the tiles
 };
 
 namespace {
@@ -816,9 +831,31 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
   P.eval_freq = p->evaluation_frequency;
   P.mode = 0;
   P.certificates = certificates ? 1 : 0;
-  int budget = p->chunk_iterations > 0 ? p->chunk_iterations : 1024;
   const size_t nlp = idx.size();
   std::vector<SolveState> states(nlp);
+  for (auto& st : states) st.status = -1;
+  // Iterations per launch. Without a time limit the chunk length follows a
+  // cost model of the LPs still running (about 50 ms at 3 TB/s, never under
+  // 30 us per iteration), so launch boundaries -- and with them the points
+  // where finished LPs hand their workgroups over -- fall on the same
+  // iterations every run and batched results are reproducible. Under a time
+  // limit the length follows the measured launch time instead.
+  const bool timed_chunks = p->chunk_iterations == 0 && p->time_limit_ms >= 0;
+  auto model_budget = [&]() {
+    double bytes = 0;
+    for (size_t k = 0; k < nlp; ++k) {
+      if (states[k].status >= 0) continue;
+      const HostLp& h = jobs[idx[k]].h;
+      const double Z = (double)h.nnz, n = h.n, m = h.m, sb = (double)esize(prec);
+      bytes += 2 * Z * (sb + 4) + 4 * (m + n + 2) + sb * (13 * n + 11 * m);
+    }
+    const double us = std::max(30.0, bytes / 3.0e6);
+    const int64_t f = std::max(1, p->evaluation_frequency);
+    int64_t b = (int64_t)(50000.0 / us);
+    b = std::max<int64_t>(f, std::min<int64_t>(1000000, (b + f - 1) / f * f));
+    return (int)b;
+  };
+  int budget = p->chunk_iterations > 0 ? p->chunk_iterations : (timed_chunks ? 1024 : model_budget());
   std::vector<double> done_ms(nlp, -1.0);
   int running_groups = (int)nlp;
   auto t_solve = clk::now();
@@ -929,13 +966,14 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
       const int rc = regroup();
       if (rc) return rc;
     }
-    if (p->chunk_iterations == 0) {
-      // aim for launches of roughly 50 ms (shorter under a time limit)
-      double target = 50.0;
-      if (p->time_limit_ms >= 0) target = std::min(target, std::max(1.0, p->time_limit_ms / 8.0));
+    if (timed_chunks) {
+      // launches of roughly 50 ms, shorter under a short time limit
+      const double target = std::min(50.0, std::max(1.0, p->time_limit_ms / 8.0));
       const double per_it = launch_ms / std::max(budget, 1);
       int nb = (int)std::min(1e7, std::max((double)p->evaluation_frequency, target / per_it));
       budget = std::max(nb, p->evaluation_frequency);
+    } else if (p->chunk_iterations == 0) {
+      budget = model_budget();
     }
   }
   const double solve_ms = ms_since(t_solve);
@@ -1008,6 +1046,17 @@ int run_jobs(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const prima_
   std::vector<int> wave;
   int used = 0;
   for (size_t j = 0; j < jobs.size(); ++j) {
+    if (jobs[j].h.n == 0 && jobs[j].h.m == 0) {
+      // nothing to iterate on: the first evaluation of Pdhg.solve finds every
+      // residual zero and both objectives at the offset (Pdhg.scala:209-212)
+      prima_result& r = *jobs[j].res;
+      std::memset(&r, 0, sizeof(r));
+      r.status = PRIMA_OPTIMAL;
+      r.objective = r.dual_objective = jobs[j].lp->offset;
+      r.final_step_size = NAN;
+      r.final_primal_weight = NAN;
+      continue;
+    }
     if (used + jobs[j].h.G > capacity && !wave.empty()) {
       int rc = run_wave(ctx, prec, jobs, wave, p, certificates, capacity);
       if (rc) return rc;
@@ -1151,21 +1200,30 @@ int mixed_batch(prima_ctx* ctx, int32_t count, const prima_lp* lps, const prima_
     which.push_back(k);
   }
   if (which.empty()) return PRIMA_OK;
+  // one time limit for the whole refinement batch: what is left after the
+  // slowest reduced pass (MixedPrecision.scala charges each LP its own)
   prima_params fp = *p;
-  std::vector<SolveJob> jobs(which.size());
-  for (size_t j = 0; j < which.size(); ++j) {
-    const int k = which[j];
-    if (p->time_limit_ms >= 0)
-      fp.time_limit_ms = std::max<int64_t>(1, p->time_limit_ms - (int64_t)red[k].solve_time_ms);
-    int rc = prepare_job(ctx, jobs[j], &lps[k], &fp, &rw[k]);
-    if (rc) return rc;
-    jobs[j].res = &results[k];
-    jobs[j].x_out = xo ? xo[k] : nullptr;
-    jobs[j].y_out = yo ? yo[k] : nullptr;
-    jobs[j].rc_out = rco ? rco[k] : nullptr;
+  if (p->time_limit_ms >= 0) {
+    double spent = 0;
+    for (int k : which) spent = std::max(spent, red[k].solve_time_ms);
+    fp.time_limit_ms = std::max<int64_t>(1, p->time_limit_ms - (int64_t)spent);
   }
-  int rc = run_jobs(ctx, 64, jobs, &fp);
-  if (rc) return rc;
+  const bool stall = mx->stall_iterations > 0 && mx->stall_iterations < p->max_iterations;
+  if (stall) fp.max_iterations = mx->stall_iterations;
+  {
+    std::vector<SolveJob> jobs(which.size());
+    for (size_t j = 0; j < which.size(); ++j) {
+      const int k = which[j];
+      int rc = prepare_job(ctx, jobs[j], &lps[k], &fp, &rw[k]);
+      if (rc) return rc;
+      jobs[j].res = &results[k];
+      jobs[j].x_out = xo ? xo[k] : nullptr;
+      jobs[j].y_out = yo ? yo[k] : nullptr;
+      jobs[j].rc_out = rco ? rco[k] : nullptr;
+    }
+    int rc = run_jobs(ctx, 64, jobs, &fp);
+    if (rc) return rc;
+  }
   for (size_t j = 0; j < which.size(); ++j) {
     const int k = which[j];
     infos[k].refinement_iterations = results[k].iterations;
@@ -1173,18 +1231,52 @@ int mixed_batch(prima_ctx* ctx, int32_t count, const prima_lp* lps, const prima_
     infos[k].refined = 1;
     results[k].total_time_ms += red[k].total_time_ms;
   }
+  if (!stall) return PRIMA_OK;
+  // warm-started passes that stalled: a cold float64 pass with the rest of
+  // the iteration budget
+  std::vector<int> cold;
+  for (int k : which)
+    if (results[k].status == PRIMA_ITERATION_LIMIT) cold.push_back(k);
+  if (cold.empty()) return PRIMA_OK;
+  prima_params cp = fp;
+  cp.max_iterations = p->max_iterations - mx->stall_iterations;
+  std::vector<prima_result> res2(cold.size());
+  std::vector<SolveJob> jobs(cold.size());
+  const prima_warm none{nullptr, nullptr, NAN, NAN};
+  for (size_t j = 0; j < cold.size(); ++j) {
+    const int k = cold[j];
+    int rc = prepare_job(ctx, jobs[j], &lps[k], &cp, &none);
+    if (rc) return rc;
+    jobs[j].res = &res2[j];
+    jobs[j].x_out = xo ? xo[k] : nullptr;
+    jobs[j].y_out = yo ? yo[k] : nullptr;
+    jobs[j].rc_out = rco ? rco[k] : nullptr;
+  }
+  int rc = run_jobs(ctx, 64, jobs, &cp);
+  if (rc) return rc;
+  for (size_t j = 0; j < cold.size(); ++j) {
+    const int k = cold[j];
+    const double before = results[k].total_time_ms;
+    infos[k].fallback_iterations = res2[j].iterations;
+    results[k] = res2[j];
+    results[k].total_time_ms += before;
+  }
   return PRIMA_OK;
 }
 
 // The seam SpMV in a given schedule: 0 picks a sub-wave width from the mean
 // row length, 1..7 are sub-waves of 1, 2, ..., 64 lanes per row, 8 is the
-// row-block (CSR-Adaptive) kernel the solver uses.
+// row-block (CSR-Adaptive) kernel the solver uses, 9 the column-panel tiles.
 hipError_t spmv_scheduled(prima_ctx* ctx, const prima_mat* a, const void* x, void* out, int sch) {
   if (sch == 0)
     return op_spmv(ctx->precision, a->rows, a->nnz, a->ptr, a->col, a->val, x, out, ctx->stream);
   if (sch == PRIMA_SPMV_ROW_BLOCKS)
     return op_spmv_blocks(ctx->precision, a->nblocks, 1, a->ptr, a->col, a->val, a->rbd, x, out,
                           ctx->stream);
+  if (sch == PRIMA_SPMV_PANELS)
+    return op_spmv_panels(ctx->precision, a->panels, a->nranges, a->range_row0, a->tile_off,
+                          a->starts, a->pcol, a->pval, a->nlong, a->long_rows, a->ptr, a->col,
+                          a->val, x, out, ctx->stream);
   return op_spmv_vector(ctx->precision, 1 << (sch - 1), a->rows, a->ptr, a->col, a->val, x, out,
                         ctx->stream);
 }
@@ -1947,6 +2039,113 @@ int prima_vec_free(prima_ctx* ctx, prima_vec* v) {
   return PRIMA_OK;
 }
 
+// Column-panel layout of a CSR matrix (PRIMA_SPMV_PANELS). Panels are 1 MiB
+// of x wide; rows are grouped greedily into ranges of at most panel_rows()
+// rows whose tiles each hold at most panel_tile() entries; a range's entries
+// are stored panel by panel, each tile row by row in the row's stored order.
+// Rows longer than 512 entries stay out of the tiles.
+static int build_panels(prima_ctx* ctx, prima_mat* M, const int32_t* row_ptr, const int32_t* col,
+                        const double* vals) {
+  const int prec = ctx->precision;
+  const int64_t W = prec == 32 ? (int64_t{1} << 18) : (int64_t{1} << 17);
+  if ((int64_t)M->cols < 4 * W) return PRIMA_OK;
+  const int P = (int)((M->cols + W - 1) / W);
+  const int R = panel_rows(), TP = panel_tile();
+  const int LONG = 512;
+  const int rows = M->rows;
+  std::vector<int32_t> range_row0{0}, long_rows;
+  std::vector<int32_t> cnt(P), rc(P);
+  std::vector<char> is_long((size_t)rows, 0);
+  int r = 0;
+  while (r < rows) {
+    std::fill(cnt.begin(), cnt.end(), 0);
+    int nr = 0;
+    while (r < rows && nr < R) {
+      const int len = row_ptr[r + 1] - row_ptr[r];
+      bool lg = len > LONG;
+      if (!lg) {
+        std::fill(rc.begin(), rc.end(), 0);
+        for (int32_t q = row_ptr[r]; q < row_ptr[r + 1]; ++q) rc[col[q] / W]++;
+        bool fits = true;
+        for (int p = 0; p < P; ++p) fits &= cnt[p] + rc[p] <= TP;
+        if (!fits && nr > 0) break;  // close the range; the row opens the next one
+        if (!fits) return fail(ctx, PRIMA_EINVAL, "row does not fit a panel tile");
+        for (int p = 0; p < P; ++p) cnt[p] += rc[p];
+      } else {
+        long_rows.push_back(r);
+        is_long[r] = 1;
+      }
+      ++r;
+      ++nr;
+    }
+    range_row0.push_back(r);
+  }
+  const int G = (int)range_row0.size() - 1;
+  std::vector<int32_t> tile_off((size_t)G * P + 1);
+  std::vector<uint16_t> starts((size_t)G * P * (R + 1), 0);
+  std::vector<int32_t> pcol;
+  std::vector<double> pv;
+  pcol.reserve((size_t)M->nnz);
+  pv.reserve((size_t)M->nnz);
+  std::vector<int32_t> counts, pos;
+  int64_t total = 0;
+  for (int g = 0; g < G; ++g) {
+    const int r0 = range_row0[g], nr = range_row0[g + 1] - r0;
+    counts.assign((size_t)P * nr, 0);
+    for (int rl = 0; rl < nr; ++rl) {
+      if (is_long[r0 + rl]) continue;
+      for (int32_t q = row_ptr[r0 + rl]; q < row_ptr[r0 + rl + 1]; ++q)
+        counts[(size_t)(col[q] / W) * nr + rl]++;
+    }
+    pos.assign((size_t)P * nr, 0);
+    for (int p = 0; p < P; ++p) {
+      const size_t t = (size_t)g * P + p;
+      tile_off[t] = (int32_t)total;
+      uint16_t* st = &starts[t * (R + 1)];
+      int run = 0;
+      for (int rl = 0; rl < nr; ++rl) {
+        st[rl] = (uint16_t)run;
+        pos[(size_t)p * nr + rl] = (int32_t)(total + run);
+        run += counts[(size_t)p * nr + rl];
+      }
+      st[nr] = (uint16_t)run;
+      total += run;
+    }
+    if (total > INT32_MAX) return fail(ctx, PRIMA_EINVAL, "matrix too large for panels");
+    pcol.resize((size_t)total);
+    pv.resize((size_t)total);
+    for (int rl = 0; rl < nr; ++rl) {
+      if (is_long[r0 + rl]) continue;
+      for (int32_t q = row_ptr[r0 + rl]; q < row_ptr[r0 + rl + 1]; ++q) {
+        int32_t& at = pos[(size_t)(col[q] / W) * nr + rl];
+        pcol[at] = col[q];
+        pv[at] = vals[q];
+        ++at;
+      }
+    }
+  }
+  tile_off[(size_t)G * P] = (int32_t)total;
+  const size_t s = esize(prec);
+  std::vector<char> pval_t((size_t)total * s);
+  to_prec(prec, pv.data(), total, pval_t.data());
+  auto up = [&](void** d, const void* h, size_t bytes) -> hipError_t {
+    hipError_t e = hipMalloc(d, std::max<size_t>(bytes, 4));
+    if (e == hipSuccess && bytes) e = hipMemcpy(*d, h, bytes, hipMemcpyHostToDevice);
+    return e;
+  };
+  hipError_t e = up((void**)&M->range_row0, range_row0.data(), range_row0.size() * 4);
+  if (e == hipSuccess) e = up((void**)&M->tile_off, tile_off.data(), tile_off.size() * 4);
+  if (e == hipSuccess) e = up((void**)&M->starts, starts.data(), starts.size() * 2);
+  if (e == hipSuccess) e = up((void**)&M->pcol, pcol.data(), pcol.size() * 4);
+  if (e == hipSuccess) e = up(&M->pval, pval_t.data(), pval_t.size());
+  if (e == hipSuccess) e = up((void**)&M->long_rows, long_rows.data(), long_rows.size() * 4);
+  if (e != hipSuccess) return dev_fail(ctx, e, "panel layout");
+  M->panels = P;
+  M->nranges = G;
+  M->nlong = (int32_t)long_rows.size();
+  return PRIMA_OK;
+}
+
 int prima_mat_upload_csr(prima_ctx* ctx, int32_t rows, int32_t cols, int64_t nnz,
                          const int32_t* row_ptr, const int32_t* col, const double* vals,
                          prima_mat** out) {
@@ -1996,6 +2195,13 @@ int prima_mat_upload_csr(prima_ctx* ctx, int32_t rows, int32_t cols, int64_t nnz
       return dev_fail(ctx, e, "row-block descriptors");
     }
   }
+  {
+    const int prc = build_panels(ctx, M, row_ptr, col, vals);
+    if (prc) {
+      prima_mat_free(ctx, M);
+      return prc;
+    }
+  }
   *out = M;
   return PRIMA_OK;
 }
@@ -2010,6 +2216,9 @@ int prima_mat_free(prima_ctx* ctx, prima_mat* M) {
   if (M->col) (void)hipFree(M->col);
   if (M->val) (void)hipFree(M->val);
   if (M->rbd) (void)hipFree(M->rbd);
+  for (void* p : {(void*)M->range_row0, (void*)M->tile_off, (void*)M->starts, (void*)M->pcol,
+                  M->pval, (void*)M->long_rows})
+    if (p) (void)hipFree(p);
   delete M;
   return PRIMA_OK;
 }
@@ -2027,8 +2236,10 @@ int prima_spmv(prima_ctx* ctx, const prima_mat* a, const prima_vec* x, prima_vec
 int prima_mat_set_schedule(prima_ctx* ctx, prima_mat* a, int32_t schedule) {
   int rc = check_open(ctx);
   if (rc) return rc;
-  if (!a || schedule < 0 || schedule > PRIMA_SPMV_ROW_BLOCKS)
+  if (!a || schedule < 0 || schedule > PRIMA_SPMV_PANELS)
     return fail(ctx, PRIMA_EINVAL, "unknown SpMV schedule");
+  if (schedule == PRIMA_SPMV_PANELS && a->panels == 0)
+    return fail(ctx, PRIMA_EINVAL, "matrix has no column-panel layout (x fits in L2)");
   a->schedule = schedule;
   return PRIMA_OK;
 }
@@ -2052,7 +2263,8 @@ int prima_mat_autotune(prima_ctx* ctx, prima_mat* a, int32_t reps, int32_t* chos
   HIP_TRY(ctx, hipMemsetAsync(x, 0, std::max<size_t>(1, (size_t)a->cols * s), ctx->stream));
   double best = INFINITY;
   int pick = PRIMA_SPMV_ROW_BLOCKS;
-  for (int sch = PRIMA_SPMV_VECTOR_1; sch <= PRIMA_SPMV_ROW_BLOCKS; ++sch) {
+  const int last = a->panels > 0 ? PRIMA_SPMV_PANELS : PRIMA_SPMV_ROW_BLOCKS;
+  for (int sch = PRIMA_SPMV_VECTOR_1; sch <= last; ++sch) {
     HIP_TRY(ctx, spmv_scheduled(ctx, a, x, y, sch));  // warm
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     for (int r = 0; r < reps; ++r) HIP_TRY(ctx, spmv_scheduled(ctx, a, x, y, sch));

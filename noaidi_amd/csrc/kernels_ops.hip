@@ -211,6 +211,119 @@ hipError_t op_spmv_blocks(int prec, int nblocks, int bpw, const int32_t* ptr, co
   return hipGetLastError();
 }
 
+// Column-panel schedule for matrices whose x does not fit an XCD's L2. The
+// columns are cut into panels of W columns (W·sizeof(T) = 1 MiB) and the rows
+// into ranges of at most PANEL_ROWS rows; the entries of range g that fall in
+// panel p form tile (g, p), stored contiguously, with uint16 row starts
+// relative to the tile. One workgroup per range sweeps the panels in order,
+// so the workgroups resident at a time gather from one or two panels of x,
+// which stay in L2, while the tiles stream from HBM. A row's sum accumulates
+// panel by panel in registers, each panel's entries in stored order: the
+// result is deterministic. Rows longer than PANEL_LONG are left out of the
+// tiles (their sums stay 0 here) and summed afterwards by k_spmv_rows.
+constexpr int PANEL_NT = 512;
+constexpr int PANEL_ROWS = 1024;
+constexpr int PANEL_RPT = PANEL_ROWS / PANEL_NT;
+constexpr int PANEL_TILE = 4096;
+
+template <typename T>
+__global__ __launch_bounds__(PANEL_NT) void k_spmv_panels(
+    int P, const int32_t* __restrict__ range_row0, const int32_t* __restrict__ tile_off,
+    const uint16_t* __restrict__ starts, const int32_t* __restrict__ pcol,
+    const T* __restrict__ pval, const T* __restrict__ x, T* __restrict__ out) {
+  __shared__ T prod[PANEL_TILE];
+  const int g = blockIdx.x;
+  const int row0 = range_row0[g], nrows = range_row0[g + 1] - row0;
+  T acc[PANEL_RPT];
+#pragma unroll
+  for (int j = 0; j < PANEL_RPT; ++j) acc[j] = T(0);
+  for (int p = 0; p < P; ++p) {
+    const int t = g * P + p;
+    const int base = tile_off[t], tn = tile_off[t + 1] - base;
+    int k = threadIdx.x;
+    for (; k + 3 * PANEL_NT < tn; k += 4 * PANEL_NT) {
+      int c[4];
+      T v[4], xv[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) c[j] = __builtin_nontemporal_load(&pcol[base + k + j * PANEL_NT]);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) v[j] = __builtin_nontemporal_load(&pval[base + k + j * PANEL_NT]);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) xv[j] = x[c[j]];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) prod[k + j * PANEL_NT] = v[j] * xv[j];
+    }
+    for (; k < tn; k += PANEL_NT)
+      prod[k] = __builtin_nontemporal_load(&pval[base + k]) *
+                x[__builtin_nontemporal_load(&pcol[base + k])];
+    __syncthreads();
+    const uint16_t* st = starts + (size_t)t * (PANEL_ROWS + 1);
+#pragma unroll
+    for (int j = 0; j < PANEL_RPT; ++j) {
+      const int r = (int)threadIdx.x + j * PANEL_NT;
+      if (r < nrows) {
+        const int e = st[r + 1];
+        for (int q = st[r]; q < e; ++q) acc[j] += prod[q];
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int j = 0; j < PANEL_RPT; ++j) {
+    const int r = (int)threadIdx.x + j * PANEL_NT;
+    if (r < nrows) out[row0 + r] = acc[j];
+  }
+}
+
+// One wave per listed row (the rows the panel tiles leave out): lanes stride
+// the row, fixed-order shuffle tree.
+template <typename T>
+__global__ __launch_bounds__(OPS_NT) void k_spmv_rows(int count, const int32_t* __restrict__ list,
+                                                      const int32_t* __restrict__ ptr,
+                                                      const int32_t* __restrict__ col,
+                                                      const T* __restrict__ val,
+                                                      const T* __restrict__ x,
+                                                      T* __restrict__ out) {
+  const int w = blockIdx.x * (OPS_NT / 64) + (int)threadIdx.x / 64;
+  const int li = threadIdx.x & 63;
+  if (w >= count) return;
+  const int r = list[w];
+  T acc = T(0);
+  for (int p = ptr[r] + li; p < ptr[r + 1]; p += 64) acc += val[p] * x[col[p]];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if (li == 0) out[r] = acc;
+}
+
+template <typename T>
+static hipError_t spmv_panels_t(int P, int nranges, const int32_t* range_row0,
+                                const int32_t* tile_off, const uint16_t* starts,
+                                const int32_t* pcol, const void* pval, int nlong,
+                                const int32_t* long_rows, const int32_t* ptr, const int32_t* col,
+                                const void* val, const void* x, void* out, hipStream_t s) {
+  if (nranges > 0)
+    k_spmv_panels<T><<<nranges, PANEL_NT, 0, s>>>(P, range_row0, tile_off, starts, pcol,
+                                                  (const T*)pval, (const T*)x, (T*)out);
+  if (nlong > 0)
+    k_spmv_rows<T><<<(nlong + OPS_NT / 64 - 1) / (OPS_NT / 64), OPS_NT, 0, s>>>(
+        nlong, long_rows, ptr, col, (const T*)val, (const T*)x, (T*)out);
+  return hipGetLastError();
+}
+
+hipError_t op_spmv_panels(int prec, int P, int nranges, const int32_t* range_row0,
+                          const int32_t* tile_off, const uint16_t* starts, const int32_t* pcol,
+                          const void* pval, int nlong, const int32_t* long_rows,
+                          const int32_t* ptr, const int32_t* col, const void* val, const void* x,
+                          void* out, hipStream_t s) {
+  return prec == 32 ? spmv_panels_t<float>(P, nranges, range_row0, tile_off, starts, pcol, pval,
+                                           nlong, long_rows, ptr, col, val, x, out, s)
+                    : spmv_panels_t<double>(P, nranges, range_row0, tile_off, starts, pcol,
+                                            pval, nlong, long_rows, ptr, col, val, x, out, s);
+}
+
+int panel_rows() { return PANEL_ROWS; }
+int panel_tile() { return PANEL_TILE; }
+
 hipError_t op_axpby(int prec, int64_t n, double a, const void* x, double b, const void* y,
                     void* out, hipStream_t s) {
   if (n == 0) return hipSuccess;

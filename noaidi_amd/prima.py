@@ -103,7 +103,7 @@ class _Mixed(C.Structure):
                 ("always_refine", C.c_int32), ("reduced_time_share", C.c_double),
                 ("refinement", C.c_int32), ("max_refinements", C.c_int32),
                 ("inner_tolerance", C.c_double), ("inner_max_iterations", C.c_int32),
-                ("pad", C.c_int32)]
+                ("stall_iterations", C.c_int32)]
 
 
 class _MixedInfo(C.Structure):
@@ -500,6 +500,7 @@ class MixedParams:
     max_refinements: int = 4
     inner_tolerance: float = 1e-4
     inner_max_iterations: int = 20_000
+    stall_iterations: int = 0  # 0: the reference contract (no cold fallback)
 
     def _c(self) -> _Mixed:
         m = _Mixed()
@@ -513,6 +514,7 @@ class MixedParams:
         m.max_refinements = self.max_refinements
         m.inner_tolerance = self.inner_tolerance
         m.inner_max_iterations = self.inner_max_iterations
+        m.stall_iterations = self.stall_iterations
         return m
 
 
@@ -582,7 +584,8 @@ def _solution(r: _Result, x, y, rc, info: Optional[_MixedInfo] = None) -> LpSolu
                        reduced_restarts=info.reduced_restarts,
                        refinement_iterations=info.refinement_iterations,
                        refined=bool(info.refined), reduced_ms=info.reduced_ms,
-                       refinement_ms=info.refinement_ms, rounds=info.rounds)
+                       refinement_ms=info.refinement_ms, rounds=info.rounds,
+                       fallback_iterations=info.fallback_iterations)
     return s
 
 
@@ -1010,6 +1013,22 @@ def unit_commitment(seed: int, buses: int = 100, chords: int = 50, snapshots: in
     costs U[100,1000], DC flows. The LP is its relaxation."""
     return _from_instance(instances_lib().prima_gen_uc(seed, buses, chords, snapshots,
                                                        gens_per_bus, min_up, min_down))
+
+
+def config3_args(k: int) -> dict:
+    """Instance k of the uneven scenario sweep: buses log-uniform in
+    [50, 2000], snapshots in {24, 72, 168}, chords N/2, storage N/4, demand
+    perturbed by U[-20%, +20%] per bus; seed 1000 + k."""
+    import random
+    r = random.Random(1000 + k)
+    buses = int(round(math.exp(r.uniform(math.log(50.0), math.log(2000.0)))))
+    snapshots = r.choice([24, 72, 168])
+    return dict(seed=1000 + k, buses=buses, chords=buses // 2, snapshots=snapshots,
+                storage=buses // 4, perturb=0.2, scenario_seed=k)
+
+
+def config3(k: int) -> LpProblem:
+    return dispatch(**config3_args(k))
 
 
 def config0(seed: int = 0) -> LpProblem:

@@ -21,21 +21,35 @@ def highs(spec):
     from scipy.optimize import linprog
     from noaidi_amd import prima
     kind, kw = spec
-    p = prima.dispatch(**kw) if kind == "dispatch" else prima.random_known(**kw)
+    if kind == "dispatch":
+        p = prima.dispatch(**kw)
+    elif kind == "sweep":
+        p = prima.config3(kw["k"])
+    else:
+        p = prima.random_known(**kw)
     A = p.to_scipy()
     ne = p.n_eq
     t = time.time()
-    r = linprog(p.c, A_eq=A[:ne] if ne else None, b_eq=p.q[:ne] if ne else None,
-                A_ub=-A[ne:] if p.m > ne else None, b_ub=-p.q[ne:] if p.m > ne else None,
-                bounds=list(zip(p.l, p.u)), method="highs",
-                options=dict(primal_feasibility_tolerance=1e-10, dual_feasibility_tolerance=1e-10))
+    method = "highs"
+    for method in ("highs", "highs-ipm"):
+        # a few instances end in HiGHS's "numerical difficulties" (status 4)
+        # with the simplex at these tolerances; the interior-point solver then
+        # settles them
+        r = linprog(p.c, A_eq=A[:ne] if ne else None, b_eq=p.q[:ne] if ne else None,
+                    A_ub=-A[ne:] if p.m > ne else None, b_ub=-p.q[ne:] if p.m > ne else None,
+                    bounds=list(zip(p.l, p.u)), method=method,
+                    options=dict(primal_feasibility_tolerance=1e-10,
+                                 dual_feasibility_tolerance=1e-10))
+        if r.status != 4:
+            break
     return dict(kind=kind, args=kw, n=p.n, m=p.m, n_eq=ne, nnz=p.nnz, status=int(r.status),
-                objective=float(r.fun) if r.status == 0 else None, seconds=time.time() - t)
+                objective=float(r.fun) if r.status == 0 else None, method=method,
+                seconds=time.time() - t)
 
 
 def main():
     specs = [("dispatch", dict(seed=1))]  # the 200-bus, 24-snapshot dispatch LP
-    specs += [("dispatch", dict(seed=1, perturb=0.2, scenario_seed=1000 + k)) for k in range(32)]
+    specs += [("dispatch", dict(seed=1, perturb=0.2, scenario_seed=1000 + k)) for k in range(256)]
     specs += [("dispatch", dict(seed=s, buses=20, chords=10, snapshots=6, storage=5))
               for s in range(8)]
     specs += [("dispatch", dict(seed=7, buses=60, chords=30, snapshots=24, storage=15))]
@@ -44,6 +58,19 @@ def main():
     with open(os.path.join(HERE, "highs_dispatch.json"), "w") as f:
         json.dump(out, f, indent=1)
     print(len(out), "instances written")
+    # the uneven sweep (instances k = 0..255 of prima.config3): the 16
+    # smallest and the 4 around the median size
+    from noaidi_amd import prima
+    size = sorted(range(256), key=lambda k: (prima.config3_args(k)["buses"] *
+                                             prima.config3_args(k)["snapshots"], k))
+    pick = size[:16] + size[126:130]
+    with ProcessPoolExecutor(8) as ex:
+        sweep = list(ex.map(highs, [("sweep", dict(k=k)) for k in pick]))
+    for d in sweep:
+        d["config3_args"] = prima.config3_args(d["args"]["k"])
+    with open(os.path.join(HERE, "highs_sweep.json"), "w") as f:
+        json.dump(sweep, f, indent=1)
+    print(len(sweep), "sweep instances written")
 
 
 if __name__ == "__main__":

@@ -219,3 +219,37 @@ def test_multi_workgroup_groups_agree(hip64, wg):
     s = hip64.solve(p, prima.PdhgParams(max_iterations=500_000, workgroups_per_lp=wg))
     assert s.status == "Optimal" and s.workgroups == wg
     assert gap(s.objective_value, d["objective"]) <= 1e-8
+
+
+def test_mixed_stall_fallback(hip32):
+    # scenario 150 of the dispatch sweep at one workgroup: the float32 pass
+    # reaches 1e-5 in 18,368 iterations, and the float64 pass warm-started
+    # from it lets the primal weight collapse and stalls (no convergence in
+    # 400k iterations; from the same start the oracle needs 69,184 and a cold
+    # device pass 69,312). With stall_iterations the stalled pass is rerun cold.
+    g = {d["args"]["scenario_seed"]: d["objective"] for d in gold(
+        lambda a: a.get("seed") == 1 and a.get("perturb") == 0.2 and "buses" not in a)}
+    p = prima.dispatch(seed=1, perturb=0.2, scenario_seed=1150)
+    s = hip32.mixed_solve(p, prima.PdhgParams(max_iterations=400_000, workgroups_per_lp=1),
+                          prima.MixedParams(stall_iterations=100_000))
+    assert s.status == "Optimal", (s.status, s.mixed)
+    assert s.mixed["fallback_iterations"] > 0
+    if 1150 in g:
+        assert gap(s.objective_value, g[1150]) <= 1e-8
+
+
+def test_uneven_sweep_batch_against_highs(hip32):
+    # instances of the uneven scenario sweep (buses log-uniform in [50, 2000],
+    # snapshots in {24, 72, 168}): the 16 smallest and one of median size
+    # (~130k variables) in one mixed-precision batch; every LP Optimal and
+    # within 1e-8 of HiGHS
+    with open(os.path.join(os.path.dirname(__file__), "golden", "highs_sweep.json")) as f:
+        gold = json.load(f)[:17]
+    probs = [prima.config3(d["args"]["k"]) for d in gold]
+    for p, d in zip(probs, gold):
+        assert (p.n, p.m, p.nnz) == (d["n"], d["m"], d["nnz"])
+    sols = hip32.solve_all(probs, prima.PdhgParams(max_iterations=1_000_000),
+                           mixed=prima.MixedParams(stall_iterations=300_000))
+    for d, s in zip(gold, sols):
+        assert s.status == "Optimal", (d["args"], s.status, s.mixed)
+        assert gap(s.objective_value, d["objective"]) <= 1e-8, (d["args"], s.objective_value)
