@@ -2047,8 +2047,11 @@ int prima_vec_free(prima_ctx* ctx, prima_vec* v) {
 static int build_panels(prima_ctx* ctx, prima_mat* M, const int32_t* row_ptr, const int32_t* col,
                         const double* vals) {
   const int prec = ctx->precision;
-  const int64_t W = prec == 32 ? (int64_t{1} << 18) : (int64_t{1} << 17);
-  if ((int64_t)M->cols < 4 * W) return PRIMA_OK;
+#ifndef PRIMA_PANEL_BYTES
+#define PRIMA_PANEL_BYTES (1 << 19)
+#endif
+  const int64_t W = PRIMA_PANEL_BYTES / (prec == 32 ? 4 : 8);
+  if ((int64_t)M->cols * (prec == 32 ? 4 : 8) < 4 * (1 << 20)) return PRIMA_OK;
   const int P = (int)((M->cols + W - 1) / W);
   const int R = panel_rows(), TP = panel_tile();
   const int LONG = 512;

@@ -212,8 +212,10 @@ hipError_t op_spmv_blocks(int prec, int nblocks, int bpw, const int32_t* ptr, co
 }
 
 // Column-panel schedule for matrices whose x does not fit an XCD's L2. The
-// columns are cut into panels of W columns (W·sizeof(T) = 1 MiB) and the rows
-// into ranges of at most PANEL_ROWS rows; the entries of range g that fall in
+// columns are cut into panels of W columns (W·sizeof(T) = 512 KiB; a sweep of
+// 512 KiB to 2 MiB panels and 512 to 2048-row ranges on a 1M x 2M matrix
+// with 20M nonzeros found this the best overall) and the rows into ranges of
+// at most PANEL_ROWS rows; the entries of range g that fall in
 // panel p form tile (g, p), stored contiguously, with uint16 row starts
 // relative to the tile. One workgroup per range sweeps the panels in order,
 // so the workgroups resident at a time gather from one or two panels of x,
@@ -221,10 +223,52 @@ hipError_t op_spmv_blocks(int prec, int nblocks, int bpw, const int32_t* ptr, co
 // panel by panel in registers, each panel's entries in stored order: the
 // result is deterministic. Rows longer than PANEL_LONG are left out of the
 // tiles (their sums stay 0 here) and summed afterwards by k_spmv_rows.
+#ifndef PRIMA_PANEL_ROWS
+#define PRIMA_PANEL_ROWS 512
+#endif
+#ifndef PRIMA_PANEL_TILE
+#define PRIMA_PANEL_TILE 2048
+#endif
 constexpr int PANEL_NT = 512;
-constexpr int PANEL_ROWS = 1024;
+constexpr int PANEL_ROWS = PRIMA_PANEL_ROWS;
 constexpr int PANEL_RPT = PANEL_ROWS / PANEL_NT;
-constexpr int PANEL_TILE = 4096;
+constexpr int PANEL_TILE = PRIMA_PANEL_TILE;
+
+constexpr int PANEL_EPT = PANEL_TILE / PANEL_NT;  // tile entries per thread
+
+// One tile's operands for this thread: column indices and values of entries
+// threadIdx.x + j*PANEL_NT, and the row starts of this thread's rows.
+template <typename T>
+struct PanelTile {
+  int n;
+  int c[PANEL_EPT];
+  T v[PANEL_EPT];
+  int s[PANEL_RPT], e[PANEL_RPT];
+};
+
+template <typename T>
+__device__ inline void panel_fetch(PanelTile<T>& f, int t, int nrows,
+                                   const int32_t* __restrict__ tile_off,
+                                   const uint16_t* __restrict__ starts,
+                                   const int32_t* __restrict__ pcol, const T* __restrict__ pval) {
+  const int base = tile_off[t];
+  f.n = tile_off[t + 1] - base;
+#pragma unroll
+  for (int j = 0; j < PANEL_EPT; ++j) {
+    const int k = (int)threadIdx.x + j * PANEL_NT;
+    if (k < f.n) {
+      f.c[j] = __builtin_nontemporal_load(&pcol[base + k]);
+      f.v[j] = __builtin_nontemporal_load(&pval[base + k]);
+    }
+  }
+  const uint16_t* st = starts + (size_t)t * (PANEL_ROWS + 1);
+#pragma unroll
+  for (int j = 0; j < PANEL_RPT; ++j) {
+    const int r = (int)threadIdx.x + j * PANEL_NT;
+    f.s[j] = r < nrows ? (int)st[r] : 0;
+    f.e[j] = r < nrows ? (int)st[r + 1] : 0;
+  }
+}
 
 template <typename T>
 __global__ __launch_bounds__(PANEL_NT) void k_spmv_panels(
@@ -237,36 +281,27 @@ __global__ __launch_bounds__(PANEL_NT) void k_spmv_panels(
   T acc[PANEL_RPT];
 #pragma unroll
   for (int j = 0; j < PANEL_RPT; ++j) acc[j] = T(0);
+  // software pipeline over the panels: the next tile's indices, values and
+  // row starts are in flight while this tile's gathers and sums run
+  PanelTile<T> cur, nxt;
+  panel_fetch(cur, g * P, nrows, tile_off, starts, pcol, pval);
   for (int p = 0; p < P; ++p) {
-    const int t = g * P + p;
-    const int base = tile_off[t], tn = tile_off[t + 1] - base;
-    int k = threadIdx.x;
-    for (; k + 3 * PANEL_NT < tn; k += 4 * PANEL_NT) {
-      int c[4];
-      T v[4], xv[4];
+    T pr[PANEL_EPT];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) c[j] = __builtin_nontemporal_load(&pcol[base + k + j * PANEL_NT]);
+    for (int j = 0; j < PANEL_EPT; ++j)
+      if ((int)threadIdx.x + j * PANEL_NT < cur.n) pr[j] = cur.v[j] * x[cur.c[j]];
+    if (p + 1 < P) panel_fetch(nxt, g * P + p + 1, nrows, tile_off, starts, pcol, pval);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) v[j] = __builtin_nontemporal_load(&pval[base + k + j * PANEL_NT]);
-#pragma unroll
-      for (int j = 0; j < 4; ++j) xv[j] = x[c[j]];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) prod[k + j * PANEL_NT] = v[j] * xv[j];
-    }
-    for (; k < tn; k += PANEL_NT)
-      prod[k] = __builtin_nontemporal_load(&pval[base + k]) *
-                x[__builtin_nontemporal_load(&pcol[base + k])];
-    __syncthreads();
-    const uint16_t* st = starts + (size_t)t * (PANEL_ROWS + 1);
-#pragma unroll
-    for (int j = 0; j < PANEL_RPT; ++j) {
-      const int r = (int)threadIdx.x + j * PANEL_NT;
-      if (r < nrows) {
-        const int e = st[r + 1];
-        for (int q = st[r]; q < e; ++q) acc[j] += prod[q];
-      }
+    for (int j = 0; j < PANEL_EPT; ++j) {
+      const int k = (int)threadIdx.x + j * PANEL_NT;
+      if (k < cur.n) prod[k] = pr[j];
     }
     __syncthreads();
+#pragma unroll
+    for (int j = 0; j < PANEL_RPT; ++j)
+      for (int q = cur.s[j]; q < cur.e[j]; ++q) acc[j] += prod[q];
+    __syncthreads();
+    cur = nxt;
   }
 #pragma unroll
   for (int j = 0; j < PANEL_RPT; ++j) {

@@ -6,10 +6,15 @@
 
 namespace prima {
 
-constexpr int NT = 512;       // threads per workgroup
+#ifndef PRIMA_NT
+#define PRIMA_NT 512
+#define PRIMA_TILE 8192
+#define PRIMA_ROWCAP 4096
+#endif
+constexpr int NT = PRIMA_NT;          // threads per workgroup
 constexpr int NWAVE = NT / 64;
-constexpr int TILE = 8192;    // nnz of a streamed row block (staged in LDS)
-constexpr int ROWCAP = 4096;  // rows of a streamed row block
+constexpr int TILE = PRIMA_TILE;      // nnz of a streamed row block (staged in LDS)
+constexpr int ROWCAP = PRIMA_ROWCAP;  // rows of a streamed row block
 constexpr int DEPTH = 16;     // independent product loads in flight per thread
 
 // ---------------------------------------------------------------------------
