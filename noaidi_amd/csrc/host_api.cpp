@@ -638,10 +638,6 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
   const size_t st_base = cv.take(idx.size() * sizeof(SolveState));
   for (size_t k = 0; k < idx.size(); ++k) W.lay[k].st = st_base + k * sizeof(SolveState);
   const size_t arena_bytes = cv.off;
-  int grid = 0;
-  for (int j : idx) grid += jobs[j].h.G;
-  W.grid = grid;
-
   auto t_up = clk::now();
   void* arena = nullptr;
   hipError_t e = hipMalloc(&arena, arena_bytes);
@@ -675,8 +671,22 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
     return put(off, conv.data(), count * s);
   };
 
+  // Continuous batching: LPs are admitted in input order while their
+  // workgroups fit the co-resident capacity; the rest wait and are admitted
+  // into the workgroups that finishing LPs free (see regroup below).
+  const size_t nlp = idx.size();
+  std::vector<char> active(nlp, 0);
+  {
+    int used = 0;
+    for (size_t k = 0; k < nlp; ++k) {
+      const int g = jobs[idx[k]].h.G;
+      if (used + g > capacity) break;
+      active[k] = 1;
+      used += g;
+    }
+  }
   int wg_base = 0;
-  W.wg_lp.resize(grid);
+  W.wg_lp.clear();
   for (size_t k = 0; k < idx.size(); ++k) {
     SolveJob& J = jobs[idx[k]];
     HostLp& h = J.h;
@@ -805,9 +815,13 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
     v.obj_norm = h.obj_norm;
     v.spec_orig = h.M->spec_orig;
     W.views.push_back(v);
-    for (int g = 0; g < h.G; ++g) W.wg_lp[wg_base + g] = (int32_t)k;
-    wg_base += h.G;
+    if (active[k]) {
+      for (int g = 0; g < h.G; ++g) W.wg_lp.push_back((int32_t)k);
+      W.views.back().wg_base = wg_base;
+      wg_base += h.G;
+    }
   }
+  W.grid = wg_base;
   HIP_TRY(ctx, hipMalloc((void**)&W.d_views, W.views.size() * sizeof(LpView)));
   HIP_TRY(ctx, hipMalloc((void**)&W.d_wg_lp,
                          std::max<size_t>(W.wg_lp.size(), (size_t)capacity) * sizeof(int32_t)));
@@ -831,7 +845,6 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
   P.eval_freq = p->evaluation_frequency;
   P.mode = 0;
   P.certificates = certificates ? 1 : 0;
-  const size_t nlp = idx.size();
   std::vector<SolveState> states(nlp);
   for (auto& st : states) st.status = -1;
   // Iterations per launch. Without a time limit the chunk length follows a
@@ -844,7 +857,7 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
   auto model_budget = [&]() {
     double bytes = 0;
     for (size_t k = 0; k < nlp; ++k) {
-      if (states[k].status >= 0) continue;
+      if (states[k].status >= 0 || !active[k]) continue;
       const HostLp& h = jobs[idx[k]].h;
       const double Z = (double)h.nnz, n = h.n, m = h.m, sb = (double)esize(prec);
       bytes += 2 * Z * (sb + 4) + 4 * (m + n + 2) + sb * (13 * n + 11 * m);
@@ -856,21 +869,41 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
     return (int)b;
   };
   int budget = p->chunk_iterations > 0 ? p->chunk_iterations : (timed_chunks ? 1024 : model_budget());
-  std::vector<double> done_ms(nlp, -1.0);
-  int running_groups = (int)nlp;
+  std::vector<double> done_ms(nlp, -1.0), admit_ms(nlp, 0.0);
+  int running_groups = 0;
+  for (size_t k = 0; k < nlp; ++k) running_groups += active[k];
   auto t_solve = clk::now();
   // Elastic batching: when LPs finish, the survivors are re-partitioned over
   // the freed workgroups. Safe at a launch boundary, where every LP's state
   // is committed; only the ownership tables and group sizes change.
   auto regroup = [&]() -> int {
-    std::vector<SolveJob> live;
     std::vector<size_t> which;
-    for (size_t k = 0; k < nlp; ++k)
-      if (states[k].status < 0) which.push_back(k);
-    if (which.empty()) return PRIMA_OK;
-    std::vector<int> newG(which.size());
-    {
-      // same policy as the initial choice, over the LPs still running
+    std::vector<size_t> waiting;
+    for (size_t k = 0; k < nlp; ++k) {
+      if (states[k].status >= 0) continue;
+      if (active[k]) which.push_back(k);
+      else waiting.push_back(k);
+    }
+    if (which.empty() && waiting.empty()) return PRIMA_OK;
+    std::vector<int> newG;
+    if (!waiting.empty()) {
+      // admit waiting LPs (input order, planned group size) into the freed
+      // workgroups; survivors keep their groups
+      int used = 0;
+      for (size_t k : which) used += jobs[idx[k]].h.G;
+      for (size_t k : which) newG.push_back(jobs[idx[k]].h.G);
+      for (size_t k : waiting) {
+        const int g = jobs[idx[k]].h.G;
+        if (used + g > capacity) break;
+        active[k] = 1;
+        admit_ms[k] = ms_since(t_solve);
+        used += g;
+        which.push_back(k);
+        newG.push_back(g);
+      }
+    } else {
+      newG.resize(which.size());
+      // no LP waits: the survivors share the device (the initial policy)
       std::vector<SolveJob> tmp(which.size());
       for (size_t j = 0; j < which.size(); ++j) {
         tmp[j].h.nnz = jobs[idx[which[j]]].h.nnz;
@@ -880,6 +913,7 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
       for (size_t j = 0; j < which.size(); ++j)
         newG[j] = std::min(tmp[j].h.G, jobs[idx[which[j]]].h.G_cap);
     }
+    if (which.empty()) return PRIMA_OK;
     bool changed = which.size() != (size_t)running_groups;
     for (size_t j = 0; j < which.size(); ++j) changed |= newG[j] != jobs[idx[which[j]]].h.G;
     if (!changed) return PRIMA_OK;
@@ -945,21 +979,30 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
     else if (p->time_limit_ms >= 0 && ms_since(t_solve) >= (double)p->time_limit_ms)
       final_status = PRIMA_TIME_LIMIT;
     if (final_status >= 0) {
+      // finalise the running LPs, then any still waiting for workgroups
+      // (those report their starting point)
       KParams F = P;
       F.mode = 1;
       F.final_status = final_status;
-      HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-      HIP_TRY(ctx, launch_pdhg(prec, W.d_views, W.d_wg_lp, W.grid, F, ctx->stream));
-      HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-      HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-      {
-        float ev_ms = 0.f;
-        HIP_TRY(ctx, hipEventElapsedTime(&ev_ms, ctx->ev0, ctx->ev1));
-        ctx->stats.kernel_ms += ev_ms;
-        ctx->stats.launches += 1;
+      for (;;) {
+        HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+        HIP_TRY(ctx, launch_pdhg(prec, W.d_views, W.d_wg_lp, W.grid, F, ctx->stream));
+        HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        {
+          float ev_ms = 0.f;
+          HIP_TRY(ctx, hipEventElapsedTime(&ev_ms, ctx->ev0, ctx->ev1));
+          ctx->stats.kernel_ms += ev_ms;
+          ctx->stats.launches += 1;
+        }
+        HIP_TRY(ctx, hipMemcpy(states.data(), W.arena + st_base, nlp * sizeof(SolveState),
+                               hipMemcpyDeviceToHost));
+        bool waiting = false;
+        for (size_t k = 0; k < nlp; ++k) waiting |= states[k].status < 0 && !active[k];
+        if (!waiting) break;
+        const int rc = regroup();
+        if (rc) return rc;
       }
-      HIP_TRY(ctx, hipMemcpy(states.data(), W.arena + st_base, nlp * sizeof(SolveState),
-                             hipMemcpyDeviceToHost));
       break;
     }
     {
@@ -1025,7 +1068,7 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
     r->dual_residual = st.kkt[1];
     r->objective = st.kkt[2];
     r->dual_objective = st.kkt[3];
-    r->solve_time_ms = done_ms[k] >= 0 ? done_ms[k] : solve_ms;
+    r->solve_time_ms = (done_ms[k] >= 0 ? done_ms[k] : solve_ms) - admit_ms[k];
     r->total_time_ms = r->solve_time_ms + upload_ms + J.prep_ms;
     r->final_step_size = st.step;
     r->final_primal_weight = st.weight;
@@ -1043,6 +1086,43 @@ int run_jobs(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const prima_
   const int occ = ctx->occ[prec == 32 ? 0 : 1];
   const int capacity = std::max(1, ctx->cu_count * occ);
   choose_groups(p, jobs, capacity);
+  // Without a time limit (whose clock starts at admission) one wave holds
+  // every LP whose arrays fit in a quarter of free device memory: LPs beyond
+  // the workgroup capacity wait and take over the workgroups of LPs that
+  // finish, so the batch has one tail instead of one per wave.
+  if (p->time_limit_ms < 0) {
+    std::vector<int> all;
+    Carver cv;
+    std::vector<LpLayout> lay;
+    std::vector<int> owner(jobs.size(), -1);
+    for (size_t j = 0; j < jobs.size(); ++j) {
+      if (jobs[j].h.n == 0 && jobs[j].h.m == 0) continue;
+      for (size_t e = 0; e < all.size(); ++e)
+        if (jobs[all[e]].h.M == jobs[j].h.M && owner[e] < 0) {
+          owner[all.size()] = (int)e;
+          break;
+        }
+      lay.push_back(carve(cv, jobs[j].h, prec, jobs[j].ylo != nullptr,
+                          owner[all.size()] >= 0 ? &lay[owner[all.size()]] : nullptr));
+      all.push_back((int)j);
+    }
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
+    bool fits = !all.empty() && cv.off <= free_b / 4;
+    for (int j : all) fits &= jobs[j].h.G <= capacity;
+    if (fits) {
+      for (size_t j = 0; j < jobs.size(); ++j)
+        if (jobs[j].h.n == 0 && jobs[j].h.m == 0) {
+          prima_result& r = *jobs[j].res;
+          std::memset(&r, 0, sizeof(r));
+          r.status = PRIMA_OPTIMAL;
+          r.objective = r.dual_objective = jobs[j].lp->offset;
+          r.final_step_size = NAN;
+          r.final_primal_weight = NAN;
+        }
+      return run_wave(ctx, prec, jobs, all, p, certificates, capacity);
+    }
+  }
   std::vector<int> wave;
   int used = 0;
   for (size_t j = 0; j < jobs.size(); ++j) {

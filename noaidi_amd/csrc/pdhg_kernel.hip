@@ -728,7 +728,15 @@ __global__ __launch_bounds__(NT) void pdhg_persistent(const LpView* __restrict__
   double* ipart = v.part;  // [2][G][kIterParts]
 
   if (P.mode == 1) {
-    if (!eval_pass<T>(v, R, false, false, S.wsum, sm)) return abort_launch(v, R, S);
+    // finalise: an LP that never ran (still waiting for workgroups) is
+    // initialised first and reports its starting point
+    const bool fresh = S.status == -2;
+    if (fresh) {
+      init_arrays<T>(v, R, sm);
+      if (!group_barrier(v.bar, G)) return abort_launch(v, R, S);
+      S.wsum = 0.0;
+    }
+    if (!eval_pass<T>(v, R, fresh, false, S.wsum, sm)) return abort_launch(v, R, S);
     return finish<T>(v, R, S, P.final_status, kkt_current(sm.E, v.offset), false);
   }
 

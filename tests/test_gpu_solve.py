@@ -253,3 +253,26 @@ def test_uneven_sweep_batch_against_highs(hip32):
     for d, s in zip(gold, sols):
         assert s.status == "Optimal", (d["args"], s.status, s.mixed)
         assert gap(s.objective_value, d["objective"]) <= 1e-8, (d["args"], s.objective_value)
+
+
+def test_batch_beyond_capacity_admits_waiting_lps(hip64):
+    # more LPs than co-resident workgroups: the extra LPs wait and take over
+    # the workgroups of LPs that finish. With one workgroup per LP every LP's
+    # arithmetic is independent of the others, so each result equals the one
+    # from a small batch, bit for bit.
+    probs = [prima.random_known(s, 120, 80, row_len=6) for s in range(300)]
+    P = prima.PdhgParams(workgroups_per_lp=1)
+    big = hip64.solve_all(probs, P)
+    pick = [0, 1, 2, 297, 298, 299]
+    small = hip64.solve_all([probs[i] for i in pick], P)
+    for i, b in zip(pick, small):
+        a = big[i]
+        assert a.status == b.status
+        assert a.iterations == b.iterations and a.objective_value == b.objective_value
+    optimal = 0
+    for p, s in zip(probs, big):
+        assert s.status in ("Optimal", "IterationLimit")
+        if s.status == "Optimal":
+            optimal += 1
+            assert gap(s.objective_value, p.known_objective) <= 1e-7
+    assert optimal >= 285
