@@ -103,40 +103,55 @@ __device__ inline void group_sum(const double* part, int stride, int G, double (
   __syncthreads();
 }
 
-// Group barrier over G co-resident workgroups (cooperative launch). The
-// waits are bounded: a workgroup that spins too long raises the abort word,
-// which every other waiter also watches, so no launch can hang.
+// Group barrier over G co-resident workgroups (cooperative launch). Each
+// workgroup publishes its stores with one release fence (L2 write-back)
+// before arriving and takes one acquire fence (L2 invalidate) after it sees
+// the generation move; the barrier words themselves are only touched with
+// read-modify-write atomics, which are coherent across XCDs without cache
+// maintenance of their own. The waits are bounded: a workgroup that spins
+// too long raises the abort word, which every other waiter also watches, so
+// no launch can hang.
 __device__ inline bool group_barrier(uint32_t* bar, int G) {
   __syncthreads();
   if (G == 1) return true;
   __shared__ int ok_sh;
   if (threadIdx.x == 0) {
     int ok = 1;
-    __threadfence();
-    uint32_t gen = __hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    uint32_t arrived = __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL,
-                                              __HIP_MEMORY_SCOPE_AGENT);
+    // a zero the compiler cannot fold, so the polls stay read-modify-writes
+    const uint32_t zero = (uint32_t)G >> 31;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    const uint32_t gen =
+        __hip_atomic_fetch_add(&bar[1], zero, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t arrived =
+        __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (arrived == (uint32_t)(G - 1)) {
-      __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&bar[1], gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      // last to arrive: take the others' releases, reset the count, then move
+      // the generation (the bump depends on the reset's returned value)
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+      const uint32_t was =
+          __hip_atomic_exchange(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(&bar[1], 1u + (was & zero), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
     } else {
       uint64_t spins = 0;
-      while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+      while (__hip_atomic_fetch_add(&bar[1], zero, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+             gen) {
         __builtin_amdgcn_s_sleep(2);
         if ((++spins & 1023u) == 0) {
-          if (__hip_atomic_load(&bar[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+          if (__hip_atomic_fetch_add(&bar[2], zero, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT) != 0u) {
             ok = 0;
             break;
           }
           if (spins > (1ull << 26)) {
-            __hip_atomic_store(&bar[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_exchange(&bar[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             ok = 0;
             break;
           }
         }
       }
     }
-    __threadfence();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     ok_sh = ok;
   }
   __syncthreads();
