@@ -329,8 +329,9 @@ void scale_vectors(const prima_lp* lp, const MatrixPrep& M, HostLp& h) {
 // Row blocks for the streamed SpMV: consecutive rows while their nnz fit the
 // LDS tile (and at most a tile's worth of rows, so empty rows cannot make a
 // block unbounded); a row longer than the tile is a block of its own.
-std::vector<int32_t> row_blocks(int32_t rows, const int32_t* ptr) {
-  const int tile = tile_nnz(), nt = block_row_cap();
+std::vector<int32_t> row_blocks(int32_t rows, const int32_t* ptr, int tile = 0, int nt = 0) {
+  if (tile <= 0) tile = tile_nnz();
+  if (nt <= 0) nt = block_row_cap();
   std::vector<int32_t> rb;
   rb.push_back(0);
   int32_t r = 0;
@@ -369,6 +370,26 @@ std::vector<int32_t> partition(const std::vector<int32_t>& rb, const int32_t* pt
   }
   w[G] = (int32_t)nb;
   return w;
+}
+
+// Row blocks for a group of G workgroups. A phase lasts as long as the
+// slowest workgroup's chain of block passes, and one pass costs about the same
+// few memory round trips whatever the block's size, so a group splits each
+// matrix into about G blocks (never beyond the coarse caps, never under 64
+// rows or 512 nonzeros); a single workgroup uses the shared coarse cut.
+void group_blocks(const MatrixPrep& M, const int32_t* aptr, int32_t n, int32_t m, int64_t nnz,
+                  int G, std::vector<int32_t>& a_rb, std::vector<int32_t>& t_rb) {
+  if (G <= 1) {
+    a_rb = M.a_rb;
+    t_rb = M.t_rb;
+    return;
+  }
+  auto cap = [&](int64_t total, int64_t lo, int64_t hi) {
+    return (int)std::max<int64_t>(lo, std::min<int64_t>(hi, (total + G - 1) / G));
+  };
+  const int tile = cap(nnz, 512, tile_nnz());
+  a_rb = row_blocks(m, aptr, tile, cap(m, 64, block_row_cap()));
+  t_rb = row_blocks(n, M.tptr.data(), tile, cap(n, 64, block_row_cap()));
 }
 
 // {first row, first nnz} pairs, one per block plus a sentinel.
@@ -534,13 +555,13 @@ LpLayout carve(Carver& cv, const HostLp& h, int prec, bool with_ylo, const LpLay
     L.a_col = cv.take(z * 4);
     L.t_ptr = cv.take((n + 1) * 4);
     L.t_col = cv.take(z * 4);
-    L.a_rb = cv.take(h.M->a_rbd.size() * 4);
-    L.t_rb = cv.take(h.M->t_rbd.size() * 4);
     L.a_val = cv.take(z * s);
     L.t_val = cv.take(z * s);
     L.rs = cv.take(m * 8);
     L.cs = cv.take(n * 8);
   }
+  L.a_rb = cv.take((m + 2) * 8);  // block descriptors: at most one block per row
+  L.t_rb = cv.take((n + 2) * 8);
   L.a_wrb = cv.take((G + 1) * 4);
   L.t_wrb = cv.take((G + 1) * 4);
   L.c = cv.take(n * s);
@@ -694,17 +715,23 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
     const LpLayout& L = W.lay[k];
     static const int32_t zero_ptr[1] = {0};
     const int32_t* aptr = lp->m > 0 ? lp->row_ptr : zero_ptr;
-    std::vector<int32_t> awrb = partition(h.M->a_rb, aptr, h.G);
-    std::vector<int32_t> twrb = partition(h.M->t_rb, h.M->tptr.data(), h.G);
-    HIP_TRY(ctx, put(L.a_wrb, awrb.data(), awrb.size() * 4));
-    HIP_TRY(ctx, put(L.t_wrb, twrb.data(), twrb.size() * 4));
+    {
+      std::vector<int32_t> arb, trb;
+      group_blocks(*h.M, aptr, h.n, h.m, h.nnz, h.G, arb, trb);
+      const std::vector<int32_t> ad = block_descriptors(arb, aptr);
+      const std::vector<int32_t> td = block_descriptors(trb, h.M->tptr.data());
+      HIP_TRY(ctx, put(L.a_rb, ad.data(), ad.size() * 4));
+      HIP_TRY(ctx, put(L.t_rb, td.data(), td.size() * 4));
+      std::vector<int32_t> awrb = partition(arb, aptr, h.G);
+      std::vector<int32_t> twrb = partition(trb, h.M->tptr.data(), h.G);
+      HIP_TRY(ctx, put(L.a_wrb, awrb.data(), awrb.size() * 4));
+      HIP_TRY(ctx, put(L.t_wrb, twrb.data(), twrb.size() * 4));
+    }
     if (owner_of[k] < 0) {  // the first LP of the wave with this matrix uploads it
       HIP_TRY(ctx, put(L.a_ptr, aptr, ((size_t)h.m + 1) * 4));
       HIP_TRY(ctx, put(L.a_col, lp->col, (size_t)h.nnz * 4));
       HIP_TRY(ctx, put(L.t_ptr, h.M->tptr.data(), ((size_t)h.n + 1) * 4));
       HIP_TRY(ctx, put(L.t_col, h.M->tcol.data(), (size_t)h.nnz * 4));
-      HIP_TRY(ctx, put(L.a_rb, h.M->a_rbd.data(), h.M->a_rbd.size() * 4));
-      HIP_TRY(ctx, put(L.t_rb, h.M->t_rbd.data(), h.M->t_rbd.size() * 4));
       HIP_TRY(ctx, put_t(L.a_val, h.M->aval.data(), (size_t)h.nnz));
       HIP_TRY(ctx, put_t(L.t_val, h.M->tval.data(), (size_t)h.nnz));
       HIP_TRY(ctx, put(L.cs, h.M->cs.data(), (size_t)h.n * 8));
@@ -923,13 +950,22 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
     for (size_t j = 0; j < which.size(); ++j) {
       const size_t k = which[j];
       SolveJob& J = jobs[idx[k]];
+      const bool regrouped = J.h.G != newG[j];
       J.h.G = newG[j];
-      static const int32_t zero_ptr[1] = {0};
-      const int32_t* aptr = J.lp->m > 0 ? J.lp->row_ptr : zero_ptr;
-      std::vector<int32_t> awrb = partition(J.h.M->a_rb, aptr, J.h.G);
-      std::vector<int32_t> twrb = partition(J.h.M->t_rb, J.h.M->tptr.data(), J.h.G);
-      HIP_TRY(ctx, put(W.lay[k].a_wrb, awrb.data(), awrb.size() * 4));
-      HIP_TRY(ctx, put(W.lay[k].t_wrb, twrb.data(), twrb.size() * 4));
+      if (regrouped) {
+        static const int32_t zero_ptr[1] = {0};
+        const int32_t* aptr = J.lp->m > 0 ? J.lp->row_ptr : zero_ptr;
+        std::vector<int32_t> arb, trb;
+        group_blocks(*J.h.M, aptr, J.h.n, J.h.m, J.h.nnz, J.h.G, arb, trb);
+        const std::vector<int32_t> ad = block_descriptors(arb, aptr);
+        const std::vector<int32_t> td = block_descriptors(trb, J.h.M->tptr.data());
+        HIP_TRY(ctx, put(W.lay[k].a_rb, ad.data(), ad.size() * 4));
+        HIP_TRY(ctx, put(W.lay[k].t_rb, td.data(), td.size() * 4));
+        std::vector<int32_t> awrb = partition(arb, aptr, J.h.G);
+        std::vector<int32_t> twrb = partition(trb, J.h.M->tptr.data(), J.h.G);
+        HIP_TRY(ctx, put(W.lay[k].a_wrb, awrb.data(), awrb.size() * 4));
+        HIP_TRY(ctx, put(W.lay[k].t_wrb, twrb.data(), twrb.size() * 4));
+      }
       W.views[k].G = J.h.G;
       W.views[k].wg_base = base;
       for (int g = 0; g < J.h.G; ++g) W.wg_lp.push_back((int32_t)k);

@@ -203,12 +203,12 @@ def test_uc_frontier_nodes_share_one_matrix(hip64):
     # the 100-bus, 24-snapshot UC relaxation and a node of its depth-9
     # frontier (same K, nine binaries fixed by bounds), HiGHS objectives to
     # 1e-8. The node is a hard LP for restarted PDHG: the oracle needs 546,688
-    # iterations for it from a cold start, the device 364,224 (measured; the
-    # trajectories part after many restarts because the device sums in a
-    # different order), the root about 119k
+    # iterations for it from a cold start; on the device the count moves
+    # between about 0.36M and more than 0.8M with the order of the global sums
+    # (which depends on the group size), the root needs about 119k
     root, g, node = mf.uc_frontier()
     assert (root.n, root.m, root.nnz) == (g["n"], g["m"], g["nnz"])
-    P = prima.PdhgParams(max_iterations=800_000, time_limit_ms=150_000)
+    P = prima.PdhgParams(max_iterations=3_000_000, time_limit_ms=200_000)
     r = hip64.solve(root, P)
     assert r.status == "Optimal"
     assert abs(r.objective_value - g["root_objective"]) / abs(g["root_objective"]) <= 1e-8
