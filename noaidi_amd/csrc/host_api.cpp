@@ -590,7 +590,7 @@ LpLayout carve(Carver& cv, const HostLp& h, int prec, bool with_ylo, const LpLay
   L.oq = cv.take(m * 8);
   L.y_out = cv.take(m * 8);
   L.part = cv.take((2 * G * kIterParts + G * kEvalParts) * 8);
-  L.bar = cv.take(16);
+  L.bar = cv.take(128 + 128 * G);  // count, generation, abort; one flag line per rank
   L.st = 0;  // placed by the wave: all states of a wave are contiguous
   L.ylo = with_ylo ? cv.take(m * s) : 0;
   L.oylo = with_ylo ? cv.take(m * 8) : 0;
@@ -605,6 +605,11 @@ struct SolveJob {
   prima_result* res = nullptr;
   double *x_out = nullptr, *y_out = nullptr, *rc_out = nullptr;
   double prep_ms = 0;
+  // iteration cap of this LP (0: the params' max_iterations); with a cold
+  // budget, a pass that hits the cap restarts cold in place with that budget
+  int32_t max_iter = 0;
+  int32_t cold_budget = 0;
+  int32_t capped_iterations = -1;  // iterations of the capped pass, once restarted
 };
 
 // Workgroups per LP. Alone, an LP is latency-bound: spread it at about 2k
@@ -692,6 +697,58 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
     return put(off, conv.data(), count * s);
   };
 
+  // Compulsory bytes of a pass (each array counted once, no gather
+  // amplification): accepted iteration 2Z(s+4) + 4(m+n+2) + s(13n+11m);
+  // rejected trial Z(s+4) + 4(m+1) + s(n+m) + s(7n+5m); checkpoint
+  // Z(s+4) + 4(m+1) + s(n+m) + 8(4n+4m).
+  auto account = [&](const SolveJob& J, const SolveState& st) {
+    const double Z = (double)J.h.nnz, n = J.h.n, m = J.h.m, sb = (double)s;
+    const double it = st.iteration, rej = (double)(st.total_trials - st.iteration);
+    const double b_iter = 2 * Z * (sb + 4) + 4 * (m + n + 2) + sb * (13 * n + 11 * m);
+    const double b_rej = Z * (sb + 4) + 4 * (m + 1) + sb * (n + m) + sb * (7 * n + 5 * m);
+    const double b_eval = Z * (sb + 4) + 4 * (m + 1) + sb * (n + m) + 8 * (4 * n + 4 * m);
+    ctx->stats.algorithmic_bytes += it * b_iter + rej * b_rej + st.evals * b_eval;
+    ctx->stats.accepted_iterations += st.iteration;
+    ctx->stats.trials += st.total_trials;
+    ctx->stats.evaluations += st.evals;
+  };
+
+  // Starting point (Pdhg.scala:265-303): x0 = proj(warm / colScale or 0),
+  // y0 = warm / rowScale with inequality multipliers clamped at 0; step from
+  // the spectral bound, weight from data norms (Pdhg.scala:250-263), warm
+  // values overriding when finite and positive.
+  auto start_point = [&](size_t k, const prima_warm* w) -> hipError_t {
+    const SolveJob& J = jobs[idx[k]];
+    const HostLp& h = J.h;
+    const LpLayout& L = W.lay[k];
+    std::vector<double> x0((size_t)h.n, 0.0), y0((size_t)h.m, 0.0);
+    if (w && w->primal)
+      for (int32_t i = 0; i < h.n; ++i) x0[i] = w->primal[i] / h.M->cs[i];
+    for (int32_t i = 0; i < h.n; ++i) {
+      const double v = x0[i];
+      x0[i] = v < h.sl[i] ? h.sl[i] : (v > h.su[i] ? h.su[i] : v);
+    }
+    if (w && w->dual)
+      for (int32_t r = 0; r < h.m; ++r) {
+        const double v = w->dual[r] / h.M->rs[r];
+        y0[r] = (r >= h.n_eq && v < 0.0) ? 0.0 : v;
+      }
+    hipError_t e = put_t(L.x, x0.data(), h.n);
+    if (e == hipSuccess) e = put_t(L.y, y0.data(), h.m);
+    SolveState st;
+    std::memset(&st, 0, sizeof(st));
+    const bool ws = w && std::isfinite(w->step_size) && w->step_size > 0.0;
+    const bool ww = w && std::isfinite(w->primal_weight) && w->primal_weight > 0.0;
+    st.step = ws ? w->step_size : (h.M->spec_scaled > 0.0 ? 1.0 / h.M->spec_scaled : 1.0);
+    const double cn = norm2(h.sc.data(), h.n), qn = norm2(h.sq.data(), h.m);
+    st.weight = ww ? w->primal_weight : ((cn > 0.0 && qn > 0.0) ? cn / qn : 1.0);
+    st.err_at_restart = INFINITY;
+    st.last_candidate = INFINITY;
+    st.status = -2;
+    if (e == hipSuccess) e = put(L.st, &st, sizeof(st));
+    return e;
+  };
+
   // Continuous batching: LPs are admitted in input order while their
   // workgroups fit the co-resident capacity; the rest wait and are admitted
   // into the workgroups that finishing LPs free (see regroup below).
@@ -755,37 +812,7 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
       HIP_TRY(ctx, put(L.oylo, yo.data(), (size_t)h.m * 8));
     }
 
-    // Starting point (Pdhg.scala:265-303): x0 = proj(warm / colScale or 0),
-    // y0 = warm / rowScale with inequality multipliers clamped at 0.
-    std::vector<double> x0((size_t)h.n, 0.0), y0((size_t)h.m, 0.0);
-    const prima_warm* w = J.warm;
-    if (w && w->primal)
-      for (int32_t i = 0; i < h.n; ++i) x0[i] = w->primal[i] / h.M->cs[i];
-    for (int32_t i = 0; i < h.n; ++i) {
-      const double v = x0[i];
-      x0[i] = v < h.sl[i] ? h.sl[i] : (v > h.su[i] ? h.su[i] : v);
-    }
-    if (w && w->dual)
-      for (int32_t r = 0; r < h.m; ++r) {
-        const double v = w->dual[r] / h.M->rs[r];
-        y0[r] = (r >= h.n_eq && v < 0.0) ? 0.0 : v;
-      }
-    HIP_TRY(ctx, put_t(L.x, x0.data(), h.n));
-    HIP_TRY(ctx, put_t(L.y, y0.data(), h.m));
-
-    // initialise: step from the spectral bound, weight from data norms
-    // (Pdhg.scala:250-263); warm values override when finite and positive.
-    SolveState st;
-    std::memset(&st, 0, sizeof(st));
-    const bool ws = w && std::isfinite(w->step_size) && w->step_size > 0.0;
-    const bool ww = w && std::isfinite(w->primal_weight) && w->primal_weight > 0.0;
-    st.step = ws ? w->step_size : (h.M->spec_scaled > 0.0 ? 1.0 / h.M->spec_scaled : 1.0);
-    const double cn = norm2(h.sc.data(), h.n), qn = norm2(h.sq.data(), h.m);
-    st.weight = ww ? w->primal_weight : ((cn > 0.0 && qn > 0.0) ? cn / qn : 1.0);
-    st.err_at_restart = INFINITY;
-    st.last_candidate = INFINITY;
-    st.status = -2;
-    HIP_TRY(ctx, put(L.st, &st, sizeof(st)));
+    HIP_TRY(ctx, start_point(k, J.warm));
 
     LpView v;
     std::memset(&v, 0, sizeof(v));
@@ -795,6 +822,7 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
     v.n_eq = h.n_eq;
     v.G = h.G;
     v.wg_base = wg_base;
+    v.max_iter = J.max_iter > 0 ? J.max_iter : p->max_iterations;
     v.a_ptr = (const int32_t*)(a + L.a_ptr);
     v.a_col = (const int32_t*)(a + L.a_col);
     v.a_val = a + L.a_val;
@@ -997,6 +1025,23 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
       HIP_TRY(ctx, hipMemcpy(states.data(), W.arena + st_base, nlp * sizeof(SolveState),
                              hipMemcpyDeviceToHost));
       for (size_t k = 0; k < nlp; ++k) {
+        // a capped pass with a cold budget restarts cold in place, keeping
+        // its workgroups, instead of after the rest of the batch
+        SolveJob& J = jobs[idx[k]];
+        if (states[k].status != PRIMA_ITERATION_LIMIT || J.cold_budget <= 0 ||
+            J.capped_iterations >= 0)
+          continue;
+        account(J, states[k]);
+        ctx->prog_base += states[k].iteration;
+        J.capped_iterations = states[k].iteration;
+        HIP_TRY(ctx, start_point(k, nullptr));
+        W.views[k].max_iter = J.cold_budget;
+        HIP_TRY(ctx, hipMemcpy(W.d_views + k, &W.views[k], sizeof(LpView),
+                               hipMemcpyHostToDevice));
+        states[k].status = -2;
+        states[k].iteration = 0;
+      }
+      for (size_t k = 0; k < nlp; ++k) {
         if (states[k].status >= 0 && done_ms[k] < 0) {
           done_ms[k] = ms_since(t_solve);
           ++newly;
@@ -1080,21 +1125,7 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
     if (J.y_out && J.h.m)
       HIP_TRY(ctx, hipMemcpy(J.y_out, W.arena + L.y_out, (size_t)J.h.m * 8,
                              hipMemcpyDeviceToHost));
-    {
-      // compulsory bytes (each array counted once, no gather amplification):
-      // accepted iteration 2Z(s+4) + 4(m+n+2) + s(13n+11m); rejected trial
-      // Z(s+4) + 4(m+1) + s(n+m) + s(7n+5m); checkpoint Z(s+4) + 4(m+1)
-      // + s(n+m) + 8(4n+4m)
-      const double Z = (double)J.h.nnz, n = J.h.n, m = J.h.m, sb = (double)s;
-      const double it = st.iteration, rej = (double)(st.total_trials - st.iteration);
-      const double b_iter = 2 * Z * (sb + 4) + 4 * (m + n + 2) + sb * (13 * n + 11 * m);
-      const double b_rej = Z * (sb + 4) + 4 * (m + 1) + sb * (n + m) + sb * (7 * n + 5 * m);
-      const double b_eval = Z * (sb + 4) + 4 * (m + 1) + sb * (n + m) + 8 * (4 * n + 4 * m);
-      ctx->stats.algorithmic_bytes += it * b_iter + rej * b_rej + st.evals * b_eval;
-      ctx->stats.accepted_iterations += st.iteration;
-      ctx->stats.trials += st.total_trials;
-      ctx->stats.evaluations += st.evals;
-    }
+    account(J, st);
     prima_result* r = J.res;
     r->status = st.status;
     r->iterations = st.iteration;
@@ -1324,58 +1355,33 @@ int mixed_batch(prima_ctx* ctx, int32_t count, const prima_lp* lps, const prima_
     for (int k : which) spent = std::max(spent, red[k].solve_time_ms);
     fp.time_limit_ms = std::max<int64_t>(1, p->time_limit_ms - (int64_t)spent);
   }
+  // A warm-started pass still unconverged after stall_iterations restarts
+  // cold with the rest of the iteration budget, in place (see run_wave).
   const bool stall = mx->stall_iterations > 0 && mx->stall_iterations < p->max_iterations;
-  if (stall) fp.max_iterations = mx->stall_iterations;
-  {
-    std::vector<SolveJob> jobs(which.size());
-    for (size_t j = 0; j < which.size(); ++j) {
-      const int k = which[j];
-      int rc = prepare_job(ctx, jobs[j], &lps[k], &fp, &rw[k]);
-      if (rc) return rc;
-      jobs[j].res = &results[k];
-      jobs[j].x_out = xo ? xo[k] : nullptr;
-      jobs[j].y_out = yo ? yo[k] : nullptr;
-      jobs[j].rc_out = rco ? rco[k] : nullptr;
-    }
-    int rc = run_jobs(ctx, 64, jobs, &fp);
-    if (rc) return rc;
-  }
+  std::vector<SolveJob> jobs(which.size());
   for (size_t j = 0; j < which.size(); ++j) {
     const int k = which[j];
-    infos[k].refinement_iterations = results[k].iterations;
-    infos[k].refinement_ms = results[k].solve_time_ms;
-    infos[k].refined = 1;
-    results[k].total_time_ms += red[k].total_time_ms;
-  }
-  if (!stall) return PRIMA_OK;
-  // warm-started passes that stalled: a cold float64 pass with the rest of
-  // the iteration budget
-  std::vector<int> cold;
-  for (int k : which)
-    if (results[k].status == PRIMA_ITERATION_LIMIT) cold.push_back(k);
-  if (cold.empty()) return PRIMA_OK;
-  prima_params cp = fp;
-  cp.max_iterations = p->max_iterations - mx->stall_iterations;
-  std::vector<prima_result> res2(cold.size());
-  std::vector<SolveJob> jobs(cold.size());
-  const prima_warm none{nullptr, nullptr, NAN, NAN};
-  for (size_t j = 0; j < cold.size(); ++j) {
-    const int k = cold[j];
-    int rc = prepare_job(ctx, jobs[j], &lps[k], &cp, &none);
+    int rc = prepare_job(ctx, jobs[j], &lps[k], &fp, &rw[k]);
     if (rc) return rc;
-    jobs[j].res = &res2[j];
+    jobs[j].res = &results[k];
     jobs[j].x_out = xo ? xo[k] : nullptr;
     jobs[j].y_out = yo ? yo[k] : nullptr;
     jobs[j].rc_out = rco ? rco[k] : nullptr;
+    if (stall) {
+      jobs[j].max_iter = mx->stall_iterations;
+      jobs[j].cold_budget = p->max_iterations - mx->stall_iterations;
+    }
   }
-  int rc = run_jobs(ctx, 64, jobs, &cp);
+  int rc = run_jobs(ctx, 64, jobs, &fp);
   if (rc) return rc;
-  for (size_t j = 0; j < cold.size(); ++j) {
-    const int k = cold[j];
-    const double before = results[k].total_time_ms;
-    infos[k].fallback_iterations = res2[j].iterations;
-    results[k] = res2[j];
-    results[k].total_time_ms += before;
+  for (size_t j = 0; j < which.size(); ++j) {
+    const int k = which[j];
+    const bool fell_back = jobs[j].capped_iterations >= 0;
+    infos[k].refinement_iterations = fell_back ? jobs[j].capped_iterations : results[k].iterations;
+    infos[k].fallback_iterations = fell_back ? results[k].iterations : 0;
+    infos[k].refinement_ms = results[k].solve_time_ms;
+    infos[k].refined = 1;
+    results[k].total_time_ms += red[k].total_time_ms;
   }
   return PRIMA_OK;
 }

@@ -103,39 +103,52 @@ __device__ inline void group_sum(const double* part, int stride, int G, double (
   __syncthreads();
 }
 
-// Group barrier over G co-resident workgroups (cooperative launch). Each
-// workgroup publishes its stores with one release fence (L2 write-back)
-// before arriving and takes one acquire fence (L2 invalidate) after it sees
-// the generation move; the barrier words themselves are only touched with
-// read-modify-write atomics, which are coherent across XCDs without cache
-// maintenance of their own. The waits are bounded: a workgroup that spins
-// too long raises the abort word, which every other waiter also watches, so
-// no launch can hang.
-__device__ inline bool group_barrier(uint32_t* bar, int G) {
+// Group barrier over the G workgroups of one LP. Words: bar[0] arrivals,
+// bar[1] completed barriers (read at launch start into *epoch), bar[2]
+// abort; bar[32 + 32 j] the release flag of rank j, each on its own 128-byte
+// line. Arrival is one read-modify-write on the count; the last to arrive
+// resets it and writes the barrier's number into every other rank's flag,
+// one lane per rank, so each waiter polls only its own line. Every poll is a
+// read-modify-write (with a zero the compiler cannot fold): those are what
+// stays coherent across XCDs without per-poll cache maintenance. The release
+// fence before arriving and the acquire fence after leaving order the data.
+// A waiter gives up after 2^26 polls (or when another sets the abort word)
+// and reports false.
+__device__ inline bool group_barrier(uint32_t* bar, int G, int rank, uint32_t* epoch) {
   __syncthreads();
   if (G == 1) return true;
   __shared__ int ok_sh;
-  if (threadIdx.x == 0) {
+  if (threadIdx.x < 64) {
     int ok = 1;
-    // a zero the compiler cannot fold, so the polls stay read-modify-writes
     const uint32_t zero = (uint32_t)G >> 31;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    const uint32_t gen =
-        __hip_atomic_fetch_add(&bar[1], zero, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const uint32_t arrived =
-        __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (arrived == (uint32_t)(G - 1)) {
-      // last to arrive: take the others' releases, reset the count, then move
-      // the generation (the bump depends on the reset's returned value)
-      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
-      const uint32_t was =
-          __hip_atomic_exchange(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_fetch_add(&bar[1], 1u + (was & zero), __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_AGENT);
-    } else {
+    uint32_t e = 0;
+    int last = 0;
+    if (threadIdx.x == 0) {
+      e = ++*epoch;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      const uint32_t arrived =
+          __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (arrived == (uint32_t)(G - 1)) {
+        // last to arrive: take the others' releases, reset the count (its
+        // return is waited for before any flag is written), count the barrier
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+        const uint32_t was =
+            __hip_atomic_exchange(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(&bar[1], 1u + (was & zero), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        last = 1 + (int)(was & zero);
+      }
+    }
+    e = __shfl(e, 0);
+    last = __shfl(last, 0);
+    if (last) {
+      for (int j = threadIdx.x; j < G; j += 64)
+        if (j != rank)
+          __hip_atomic_exchange(&bar[32 + 32 * j], e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else if (threadIdx.x == 0) {
       uint64_t spins = 0;
-      while (__hip_atomic_fetch_add(&bar[1], zero, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
-             gen) {
+      uint32_t* flag = &bar[32 + 32 * rank];
+      while (__hip_atomic_fetch_add(flag, zero, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != e) {
         __builtin_amdgcn_s_sleep(2);
         if ((++spins & 1023u) == 0) {
           if (__hip_atomic_fetch_add(&bar[2], zero, __ATOMIC_RELAXED,
@@ -151,8 +164,10 @@ __device__ inline bool group_barrier(uint32_t* bar, int G) {
         }
       }
     }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    ok_sh = ok;
+    if (threadIdx.x == 0) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      ok_sh = ok;
+    }
   }
   __syncthreads();
   return ok_sh != 0;
@@ -497,6 +512,7 @@ struct Smem {
   int rptr[ROWCAP + 1];
   double red[NWAVE * kEvalParts + kEvalParts];
   double E[kEvalParts];  // group totals of the last evaluation
+  uint32_t epoch;        // group barriers completed (thread 0's copy)
 };
 
 // One evaluation pass (Kkt.evaluate + certificate inputs) over own rows and
@@ -525,7 +541,7 @@ __device__ __noinline__ bool eval_pass(const LpView& v, const Ranges& R, bool fr
   if (threadIdx.x == 0) {
     for (int k = 0; k < E_COUNT; ++k) epart[R.rank * kEvalParts + k] = e[k];
   }
-  if (!group_barrier(v.bar, R.G)) return false;
+  if (!group_barrier(v.bar, R.G, R.rank, &sm.epoch)) return false;
   double tot[E_COUNT];
   group_sum<E_COUNT>(epart, kEvalParts, R.G, tot, sm.red);
   if (threadIdx.x == 0) {
@@ -737,6 +753,12 @@ __global__ __launch_bounds__(NT) void pdhg_persistent(const LpView* __restrict__
   R.r0 = v.a_rb[2 * R.ab0];
   R.r1 = v.a_rb[2 * R.ab1];
   const int G = R.G;
+  // barriers this LP's group has completed so far: the base of this launch's
+  // barrier numbers (read before this workgroup first arrives, so every
+  // workgroup of the group reads the same value)
+  if (threadIdx.x == 0)
+    sm.epoch = __hip_atomic_fetch_add(&v.bar[1], (uint32_t)G >> 31, __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
 
   SolveState S = *v.st;
   if (S.status >= 0) return;
@@ -748,7 +770,7 @@ __global__ __launch_bounds__(NT) void pdhg_persistent(const LpView* __restrict__
     const bool fresh = S.status == -2;
     if (fresh) {
       init_arrays<T>(v, R, sm);
-      if (!group_barrier(v.bar, G)) return abort_launch(v, R, S);
+      if (!group_barrier(v.bar, G, R.rank, &sm.epoch)) return abort_launch(v, R, S);
       S.wsum = 0.0;
     }
     if (!eval_pass<T>(v, R, fresh, false, S.wsum, sm)) return abort_launch(v, R, S);
@@ -757,7 +779,7 @@ __global__ __launch_bounds__(NT) void pdhg_persistent(const LpView* __restrict__
 
   if (S.status == -2) {
     init_arrays<T>(v, R, sm);
-    if (!group_barrier(v.bar, G)) return abort_launch(v, R, S);
+    if (!group_barrier(v.bar, G, R.rank, &sm.epoch)) return abort_launch(v, R, S);
     S.wsum = 0.0;
     if (!eval_pass<T>(v, R, true, false, 0.0, sm)) return abort_launch(v, R, S);
     const Kkt cur = kkt_current(sm.E, v.offset);
@@ -791,7 +813,7 @@ __global__ __launch_bounds__(NT) void pdhg_persistent(const LpView* __restrict__
       block_sum<1>(a, sm.red);
       if (threadIdx.x == 0) ip[0] = a[0];
     }
-    if (!group_barrier(v.bar, G)) return abort_launch(v, R, S);
+    if (!group_barrier(v.bar, G, R.rank, &sm.epoch)) return abort_launch(v, R, S);
 
     // phase 2: own rows
     {
@@ -806,7 +828,7 @@ __global__ __launch_bounds__(NT) void pdhg_persistent(const LpView* __restrict__
         ip[2] = a[1];
       }
     }
-    if (!group_barrier(v.bar, G)) return abort_launch(v, R, S);
+    if (!group_barrier(v.bar, G, R.rank, &sm.epoch)) return abort_launch(v, R, S);
 
     // decide (identical in every workgroup)
     double tot[3];
@@ -852,7 +874,7 @@ __global__ __launch_bounds__(NT) void pdhg_persistent(const LpView* __restrict__
       pending_rows = false;
       commit_columns_cold<T>(v, R, wpend, sm);
       pending = false;
-      if (!group_barrier(v.bar, G)) return abort_launch(v, R, S);
+      if (!group_barrier(v.bar, G, R.rank, &sm.epoch)) return abort_launch(v, R, S);
       const bool have_avg = S.wsum > 0.0;
       if (!eval_pass<T>(v, R, true, have_avg, S.wsum, sm)) return abort_launch(v, R, S);
       S.evals++;
@@ -863,7 +885,7 @@ __global__ __launch_bounds__(NT) void pdhg_persistent(const LpView* __restrict__
       if (st >= 0) return finish<T>(v, R, S, st, err, avg);
     }
 
-    if (S.iteration >= P.max_iter) {
+    if (S.iteration >= v.max_iter) {
       if (pending_rows) {
         commit_rows_cold<T>(v, R, wpend);
         pending_rows = false;

@@ -42,7 +42,7 @@ struct LpView {
   int32_t n, m, n_eq;
   int32_t G;        // workgroups in this group
   int32_t wg_base;  // first blockIdx.x of this group
-  int32_t pad0;
+  int32_t max_iter;  // this LP's iteration cap
   const int32_t *a_ptr, *a_col;
   const void* a_val;
   const int32_t *t_ptr, *t_col;
@@ -59,7 +59,7 @@ struct LpView {
   void *x, *y, *kx, *kty, *xn, *yn, *xbar, *kxbar, *sx, *sy, *skx, *skty, *xr, *yr;
   double *x_out, *y_out, *rc_out;
   double* part;       // 2 * G * 4 iteration partials, then G * kEvalParts
-  uint32_t* bar;      // barrier words: count, generation, abort
+  uint32_t* bar;      // barrier words: count, generation, abort, then a flag line per rank
   SolveState* st;
   double offset, rhs_norm, obj_norm, spec_orig;
 };

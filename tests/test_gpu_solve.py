@@ -238,6 +238,28 @@ def test_mixed_stall_fallback(hip32):
         assert gap(s.objective_value, g[1150]) <= 1e-8
 
 
+def test_mixed_stall_fallback_in_a_batch(hip32):
+    # the stalled warm pass restarts cold in place while the rest of the
+    # batch runs on; at one workgroup per LP every LP's arithmetic is its
+    # own, so the batched result equals the LP solved alone, bit for bit
+    P = prima.PdhgParams(max_iterations=400_000, workgroups_per_lp=1)
+    M = prima.MixedParams(stall_iterations=100_000)
+    probs = [prima.dispatch(seed=1, perturb=0.2, scenario_seed=1000 + k) for k in (150, 3, 7, 11)]
+    alone = hip32.mixed_solve(probs[0], P, M)
+    batch = hip32.solve_all(probs, P, mixed=M)
+    b = batch[0]
+    assert b.status == "Optimal" and b.mixed["fallback_iterations"] > 0
+    assert b.mixed["refinement_iterations"] == 100_000
+    assert b.mixed["fallback_iterations"] == alone.mixed["fallback_iterations"]
+    assert b.iterations == alone.iterations
+    assert np.array_equal(b.primal, alone.primal) and np.array_equal(b.dual, alone.dual)
+    for s in batch[1:]:
+        # the others may stall at one workgroup too; a restart shows as a
+        # capped warm pass
+        assert s.status == "Optimal"
+        assert s.mixed["fallback_iterations"] == 0 or s.mixed["refinement_iterations"] == 100_000
+
+
 def test_uneven_sweep_batch_against_highs(hip32):
     # instances of the uneven scenario sweep (buses log-uniform in [50, 2000],
     # snapshots in {24, 72, 168}): the 16 smallest and one of median size
