@@ -334,6 +334,12 @@ def main():
         iters += [(s.mixed["reduced_iterations"], s.mixed["refinement_iterations"],
                    s.mixed["fallback_iterations"]) for s in sols]
         log(f"step done: {time.perf_counter() - t0:.2f} s cumulative, worst gap {worst:.2e}")
+        heavy = sorted(range(B), key=lambda i: -(sols[i].mixed["refinement_iterations"]
+                                                + sols[i].mixed["fallback_iterations"]))[:8]
+        log("heaviest LPs (f32, f64, cold f64 iterations; seconds; final workgroups): " + "; ".join(
+            f"#{i} {sols[i].mixed['reduced_iterations']}/{sols[i].mixed['refinement_iterations']}"
+            f"/{sols[i].mixed['fallback_iterations']} {sols[i].total_time_ms / 1e3:.1f}s "
+            f"G{sols[i].workgroups}" for i in heavy))
     elapsed = time.perf_counter() - t0
     st = k32.stats()
     value = B * args.steps / elapsed

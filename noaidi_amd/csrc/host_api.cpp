@@ -612,12 +612,14 @@ struct SolveJob {
   int32_t capped_iterations = -1;  // iterations of the capped pass, once restarted
 };
 
-// Workgroups per LP. Alone, an LP is latency-bound: spread it at about 2k
-// nonzeros per workgroup. In a batch that would not fit the device that way,
-// every extra workgroup of an LP costs barrier latency that other LPs could
-// have used: give each LP about 16k nonzeros per workgroup, capped by its
-// share of the device (measured on the 65k-nnz dispatch LP: 32 workgroups
-// alone, 4 in a batch of 32, 1 in a batch of 256).
+// Workgroups per LP. Alone, an LP is latency-bound: spread it thin. With the
+// device at most half full, give each LP about 1k nonzeros per workgroup; up
+// to a full device, about 2k; beyond that, about 16k, capped by its share of
+// the device (measured on the 65k-nnz dispatch LP, f64 microseconds per
+// iteration: alone 21.8 at 64 workgroups against 25.4 at 32; four LPs 30.5 at
+// 32 each against 32.8 at 64; 32 LPs at 4, 256 LPs at 1). G_cap, which sizes
+// an LP's per-rank arrays and bounds later regrouping, is always the alone
+// size, so a batch's last LPs can spread out as the others finish.
 void choose_groups(const prima_params* p, std::vector<SolveJob>& jobs, int capacity) {
   if (p->workgroups_per_lp > 0) {
     for (auto& J : jobs) {
@@ -627,11 +629,20 @@ void choose_groups(const prima_params* p, std::vector<SolveJob>& jobs, int capac
     return;
   }
   auto ceil_div = [](int64_t a, int64_t b) { return (a + b - 1) / b; };
+  auto size_at = [&](const SolveJob& J, int64_t per) {
+    return std::max<int64_t>(1, std::min<int64_t>(ceil_div(std::max<int64_t>(J.h.nnz, 1), per),
+                                                  capacity));
+  };
+  int64_t fine = 0;
+  for (auto& J : jobs) {
+    const int64_t g = size_at(J, 1024);
+    J.h.G_cap = std::max(J.h.G_cap, (int)g);
+    fine += g;
+  }
+  const int64_t per = fine <= capacity / 2 ? 1024 : 2048;
   int64_t total = 0;
   for (auto& J : jobs) {
-    const int64_t g = ceil_div(std::max<int64_t>(J.h.nnz, 1), 2048);
-    J.h.G = (int)std::max<int64_t>(1, std::min<int64_t>(g, capacity));
-    J.h.G_cap = std::max(J.h.G_cap, J.h.G);
+    J.h.G = (int)size_at(J, per);
     total += J.h.G;
   }
   if (total <= capacity) return;
