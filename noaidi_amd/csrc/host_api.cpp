@@ -907,7 +907,6 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
   P.r_nec = p->restart_necessary;
   P.r_art = p->restart_artificial;
   P.theta = p->primal_weight_smoothing;
-  P.max_iter = p->max_iterations;
   P.eval_freq = p->evaluation_frequency;
   P.mode = 0;
   P.certificates = certificates ? 1 : 0;

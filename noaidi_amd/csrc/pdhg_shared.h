@@ -29,7 +29,7 @@ struct SolveState {
 struct KParams {
   double eps_abs, eps_rel, infeas_tol;
   double r_suff, r_nec, r_art, theta;
-  int32_t max_iter, eval_freq;
+  int32_t eval_freq;     // iteration caps are per LP (LpView::max_iter)
   int32_t iter_budget;   // stop after a checkpoint once this many iterations ran
   int32_t mode;          // 0: run, 1: finalize with final_status
   int32_t final_status;
