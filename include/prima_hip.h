@@ -219,9 +219,12 @@ typedef struct {
  * stall_iterations > 0 (no counterpart in the reference; 0 keeps its
  * contract) caps the warm-started float64 pass at that many iterations; an LP
  * still unconverged then gets a cold float64 pass with the rest of
- * max_iterations. A warm start can put the adaptive primal weight on a path
- * that stalls for hundreds of thousands of iterations where a cold start
- * converges. */
+ * max_iterations. Without a time limit that cold pass races the warm one:
+ * it starts once the warm pass has run a quarter of the cap while the device
+ * has workgroups to spare, and whichever concludes first (checked at launch
+ * boundaries, which fall on the same iterations every run) gives the result.
+ * A warm start can put the adaptive primal weight on a path that stalls for
+ * hundreds of thousands of iterations where a cold start converges. */
 typedef struct {
   double reduced_tolerance;
   int32_t reduced_max_iterations;

@@ -610,15 +610,20 @@ struct SolveJob {
   int32_t max_iter = 0;
   int32_t cold_budget = 0;
   int32_t capped_iterations = -1;  // iterations of the capped pass, once restarted
+  // a warm pass and its cold twin (indices into the same job list): the twin
+  // waits dormant and races the warm pass once that runs long (see run_wave)
+  int shadow = -1, shadow_of = -1;
 };
 
-// Workgroups per LP. Alone, an LP is latency-bound: spread it thin. With the
-// device at most half full, give each LP about 1k nonzeros per workgroup; up
-// to a full device, about 2k; beyond that, about 16k, capped by its share of
-// the device (measured on the 65k-nnz dispatch LP, f64 microseconds per
-// iteration: alone 21.8 at 64 workgroups against 25.4 at 32; four LPs 30.5 at
-// 32 each against 32.8 at 64; 32 LPs at 4, 256 LPs at 1). G_cap, which sizes
-// an LP's per-rank arrays and bounds later regrouping, is always the alone
+// Workgroups per LP. Alone, an LP is latency-bound: spread it at about 1k
+// nonzeros per workgroup. In a batch, filling the device pays (65k-nnz
+// dispatch LP, f64 LP-iterations/s with all 256 workgroups against half of
+// them: 128 LPs 522k at G = 2 vs 395k at 1; 64 LPs 451k at 4 vs 381k at 2;
+// 32 LPs 343k at 8 vs 331k at 4; 16 LPs 295k at 16 vs 237k at 8), except
+// that four or more LPs gain nothing beyond 32 workgroups each (4 LPs: 30.8
+// us per iteration at 32, 32.8 at 64). So each LP gets its share of the
+// device, capped by its alone size and, from four LPs on, by 32. G_cap, which
+// sizes an LP's per-rank arrays and bounds later regrouping, is the alone
 // size, so a batch's last LPs can spread out as the others finish.
 void choose_groups(const prima_params* p, std::vector<SolveJob>& jobs, int capacity) {
   if (p->workgroups_per_lp > 0) {
@@ -628,28 +633,16 @@ void choose_groups(const prima_params* p, std::vector<SolveJob>& jobs, int capac
     }
     return;
   }
-  auto ceil_div = [](int64_t a, int64_t b) { return (a + b - 1) / b; };
-  auto size_at = [&](const SolveJob& J, int64_t per) {
-    return std::max<int64_t>(1, std::min<int64_t>(ceil_div(std::max<int64_t>(J.h.nnz, 1), per),
-                                                  capacity));
-  };
-  int64_t fine = 0;
+  int64_t B = 0;  // cold twins (run_wave) do not count: they only race a warm pass
+  for (const auto& J : jobs) B += J.shadow_of < 0;
+  const int64_t share = std::max<int64_t>(1, capacity / std::max<int64_t>(B, 1));
   for (auto& J : jobs) {
-    const int64_t g = size_at(J, 1024);
-    J.h.G_cap = std::max(J.h.G_cap, (int)g);
-    fine += g;
-  }
-  const int64_t per = fine <= capacity / 2 ? 1024 : 2048;
-  int64_t total = 0;
-  for (auto& J : jobs) {
-    J.h.G = (int)size_at(J, per);
-    total += J.h.G;
-  }
-  if (total <= capacity) return;
-  const int64_t share = std::max<int64_t>(1, capacity / (int64_t)jobs.size());
-  for (auto& J : jobs) {
-    const int64_t g = ceil_div(std::max<int64_t>(J.h.nnz, 1), 16384);
-    J.h.G = (int)std::max<int64_t>(1, std::min(g, share));
+    const int64_t nnz = std::max<int64_t>(J.h.nnz, 1);
+    const int64_t alone = std::max<int64_t>(1, std::min<int64_t>((nnz + 1023) / 1024, capacity));
+    J.h.G_cap = std::max(J.h.G_cap, (int)alone);
+    int64_t g = std::min(alone, share);
+    if (B >= 4) g = std::min<int64_t>(g, 32);
+    J.h.G = (int)std::max<int64_t>(1, g);
   }
 }
 
@@ -765,9 +758,18 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
   // into the workgroups that finishing LPs free (see regroup below).
   const size_t nlp = idx.size();
   std::vector<char> active(nlp, 0);
+  // cold twins of warm passes wait dormant until they race (see below);
+  // dropped slots ran but do not report
+  std::vector<char> dormant(nlp, 0), dropped(nlp, 0);
+  std::vector<int> slot_of(jobs.size(), -1);
+  for (size_t k = 0; k < nlp; ++k) {
+    slot_of[idx[k]] = (int)k;
+    dormant[k] = jobs[idx[k]].shadow_of >= 0;
+  }
   {
     int used = 0;
     for (size_t k = 0; k < nlp; ++k) {
+      if (dormant[k]) continue;
       const int g = jobs[idx[k]].h.G;
       if (used + g > capacity) break;
       active[k] = 1;
@@ -945,7 +947,7 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
     std::vector<size_t> which;
     std::vector<size_t> waiting;
     for (size_t k = 0; k < nlp; ++k) {
-      if (states[k].status >= 0) continue;
+      if (states[k].status >= 0 || dormant[k]) continue;
       if (active[k]) which.push_back(k);
       else waiting.push_back(k);
     }
@@ -1034,12 +1036,63 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
       int64_t wave_iters = 0, newly = 0;
       HIP_TRY(ctx, hipMemcpy(states.data(), W.arena + st_base, nlp * sizeof(SolveState),
                              hipMemcpyDeviceToHost));
+      // Warm passes with a cold twin: once the warm pass has run a quarter of
+      // its cap while fewer than capacity/32 LPs run (workgroups to spare),
+      // the twin starts; the first of the two to reach a conclusive status at
+      // a launch boundary stands for the LP and the other stops where it is.
+      // A warm pass that reaches its cap leaves the LP to the twin (started
+      // then if it was not yet). Boundaries fall on the same iterations every
+      // run, so the outcome is reproducible.
+      {
+        int running = 0;
+        for (size_t k = 0; k < nlp; ++k)
+          running += active[k] && !dormant[k] && states[k].status < 0;
+        auto conclusive_status = [](int st) {
+          return st == PRIMA_OPTIMAL || st == PRIMA_PRIMAL_INFEASIBLE ||
+                 st == PRIMA_DUAL_INFEASIBLE;
+        };
+        auto stop = [&](size_t k) -> hipError_t {
+          dropped[k] = 1;
+          states[k].status = PRIMA_INTERRUPTED;
+          return put(W.lay[k].st + offsetof(SolveState, status), &states[k].status,
+                     sizeof(int32_t));
+        };
+        for (size_t k = 0; k < nlp; ++k) {
+          SolveJob& J = jobs[idx[k]];
+          if (J.shadow < 0 || dropped[k]) continue;
+          const size_t t = (size_t)slot_of[J.shadow];
+          if (states[k].status >= 0) {
+            if (states[k].status == PRIMA_ITERATION_LIMIT) {
+              J.capped_iterations = states[k].iteration;
+              dropped[k] = 1;
+              dormant[t] = 0;  // the twin stands in (waiting, if it never ran)
+            } else if (!dropped[t]) {
+              // the warm pass concluded: the twin is not needed (a finished
+              // twin keeps its state, a running or dormant one is stopped)
+              dormant[t] = 0;
+              if (states[t].status >= 0) dropped[t] = 1;
+              else HIP_TRY(ctx, stop(t));
+            }
+            continue;
+          }
+          if (dormant[t] && states[k].iteration >= J.max_iter / 4 &&
+              running < capacity / 32) {
+            dormant[t] = 0;
+            ++running;
+          }
+          if (!dormant[t] && !dropped[t] && states[t].status >= 0 &&
+              conclusive_status(states[t].status)) {
+            J.capped_iterations = states[k].iteration;
+            HIP_TRY(ctx, stop(k));
+          }
+        }
+      }
       for (size_t k = 0; k < nlp; ++k) {
         // a capped pass with a cold budget restarts cold in place, keeping
         // its workgroups, instead of after the rest of the batch
         SolveJob& J = jobs[idx[k]];
         if (states[k].status != PRIMA_ITERATION_LIMIT || J.cold_budget <= 0 ||
-            J.capped_iterations >= 0)
+            J.capped_iterations >= 0 || J.shadow >= 0)
           continue;
         account(J, states[k]);
         ctx->prog_base += states[k].iteration;
@@ -1089,7 +1142,8 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
         HIP_TRY(ctx, hipMemcpy(states.data(), W.arena + st_base, nlp * sizeof(SolveState),
                                hipMemcpyDeviceToHost));
         bool waiting = false;
-        for (size_t k = 0; k < nlp; ++k) waiting |= states[k].status < 0 && !active[k];
+        for (size_t k = 0; k < nlp; ++k)
+          waiting |= states[k].status < 0 && !active[k] && !dormant[k];
         if (!waiting) break;
         const int rc = regroup();
         if (rc) return rc;
@@ -1125,6 +1179,13 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
     SolveJob& J = jobs[idx[k]];
     const LpLayout& L = W.lay[k];
     const SolveState& st = states[k];
+    // a slot reports unless it was dropped, never had to run, or is a twin
+    // whose warm pass still stands (an interrupted race)
+    const bool twin_unneeded = J.shadow_of >= 0 && !dropped[(size_t)slot_of[J.shadow_of]];
+    if (dropped[k] || dormant[k] || twin_unneeded) {
+      if (!dormant[k]) account(J, st);
+      continue;
+    }
     if (st.status < 0) return fail(ctx, PRIMA_EDEVICE, "solve ended without a status");
     if (J.x_out && J.h.n)
       HIP_TRY(ctx, hipMemcpy(J.x_out, W.arena + L.x_out, (size_t)J.h.n * 8,
@@ -1145,7 +1206,9 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
     r->dual_residual = st.kkt[1];
     r->objective = st.kkt[2];
     r->dual_objective = st.kkt[3];
-    r->solve_time_ms = (done_ms[k] >= 0 ? done_ms[k] : solve_ms) - admit_ms[k];
+    // a twin reports the LP's time from the warm pass's admission
+    const size_t a = J.shadow_of >= 0 ? (size_t)slot_of[J.shadow_of] : k;
+    r->solve_time_ms = (done_ms[k] >= 0 ? done_ms[k] : solve_ms) - admit_ms[a];
     r->total_time_ms = r->solve_time_ms + upload_ms + J.prep_ms;
     r->final_step_size = st.step;
     r->final_primal_weight = st.weight;
@@ -1200,9 +1263,17 @@ int run_jobs(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const prima_
       return run_wave(ctx, prec, jobs, all, p, certificates, capacity);
     }
   }
+  // Waves cannot race a warm pass against a twin in another wave: a warm
+  // pass restarts cold in place instead, and the twins are left out.
+  for (auto& J : jobs)
+    if (J.shadow >= 0) {
+      J.cold_budget = jobs[(size_t)J.shadow].max_iter;
+      J.shadow = -1;
+    }
   std::vector<int> wave;
   int used = 0;
   for (size_t j = 0; j < jobs.size(); ++j) {
+    if (jobs[j].shadow_of >= 0) continue;
     if (jobs[j].h.n == 0 && jobs[j].h.m == 0) {
       // nothing to iterate on: the first evaluation of Pdhg.solve finds every
       // residual zero and both objectives at the offset (Pdhg.scala:209-212)
@@ -1368,23 +1439,34 @@ int mixed_batch(prima_ctx* ctx, int32_t count, const prima_lp* lps, const prima_
   // A warm-started pass still unconverged after stall_iterations restarts
   // cold with the rest of the iteration budget, in place (see run_wave).
   const bool stall = mx->stall_iterations > 0 && mx->stall_iterations < p->max_iterations;
-  std::vector<SolveJob> jobs(which.size());
-  for (size_t j = 0; j < which.size(); ++j) {
-    const int k = which[j];
-    int rc = prepare_job(ctx, jobs[j], &lps[k], &fp, &rw[k]);
+  // Without a time limit each warm pass gets a cold twin that races it once
+  // it runs long (run_wave); under one, the warm pass restarts in place.
+  const size_t N = which.size();
+  const bool twins = stall && fp.time_limit_ms < 0;
+  std::vector<SolveJob> jobs(twins ? 2 * N : N);
+  const prima_warm none{nullptr, nullptr, NAN, NAN};
+  for (size_t j = 0; j < jobs.size(); ++j) {
+    const int k = which[j % N];
+    const bool twin = j >= N;
+    int rc = prepare_job(ctx, jobs[j], &lps[k], &fp, twin ? &none : &rw[k]);
     if (rc) return rc;
     jobs[j].res = &results[k];
     jobs[j].x_out = xo ? xo[k] : nullptr;
     jobs[j].y_out = yo ? yo[k] : nullptr;
     jobs[j].rc_out = rco ? rco[k] : nullptr;
-    if (stall) {
+    if (!stall) continue;
+    if (twin) {
+      jobs[j].max_iter = p->max_iterations - mx->stall_iterations;
+      jobs[j].shadow_of = (int)(j - N);
+    } else {
       jobs[j].max_iter = mx->stall_iterations;
-      jobs[j].cold_budget = p->max_iterations - mx->stall_iterations;
+      if (twins) jobs[j].shadow = (int)(j + N);
+      else jobs[j].cold_budget = p->max_iterations - mx->stall_iterations;
     }
   }
   int rc = run_jobs(ctx, 64, jobs, &fp);
   if (rc) return rc;
-  for (size_t j = 0; j < which.size(); ++j) {
+  for (size_t j = 0; j < N; ++j) {
     const int k = which[j];
     const bool fell_back = jobs[j].capped_iterations >= 0;
     infos[k].refinement_iterations = fell_back ? jobs[j].capped_iterations : results[k].iterations;

@@ -238,26 +238,27 @@ def test_mixed_stall_fallback(hip32):
         assert gap(s.objective_value, g[1150]) <= 1e-8
 
 
-def test_mixed_stall_fallback_in_a_batch(hip32):
-    # the stalled warm pass restarts cold in place while the rest of the
-    # batch runs on; at one workgroup per LP every LP's arithmetic is its
-    # own, so the batched result equals the LP solved alone, bit for bit
+def test_mixed_stall_fallback_in_a_batch(hip32, hip64):
+    # the stalled warm pass is raced by a cold twin (started once the warm
+    # pass has run a quarter of its cap with workgroups to spare); the twin
+    # converges first and stands for the LP. At one workgroup per LP every
+    # LP's arithmetic is its own, so the twin's result equals the LP's cold
+    # float64 solve, and the batched result the LP solved alone, bit for bit.
     P = prima.PdhgParams(max_iterations=400_000, workgroups_per_lp=1)
     M = prima.MixedParams(stall_iterations=100_000)
     probs = [prima.dispatch(seed=1, perturb=0.2, scenario_seed=1000 + k) for k in (150, 3, 7, 11)]
     alone = hip32.mixed_solve(probs[0], P, M)
     batch = hip32.solve_all(probs, P, mixed=M)
+    cold = hip64.solve(probs[0], prima.PdhgParams(max_iterations=300_000, workgroups_per_lp=1))
     b = batch[0]
     assert b.status == "Optimal" and b.mixed["fallback_iterations"] > 0
-    assert b.mixed["refinement_iterations"] == 100_000
-    assert b.mixed["fallback_iterations"] == alone.mixed["fallback_iterations"]
-    assert b.iterations == alone.iterations
+    assert 25_000 <= b.mixed["refinement_iterations"] <= 100_000
+    assert b.mixed["fallback_iterations"] == alone.mixed["fallback_iterations"] == cold.iterations
+    assert b.iterations == alone.iterations == cold.iterations
     assert np.array_equal(b.primal, alone.primal) and np.array_equal(b.dual, alone.dual)
+    assert np.array_equal(b.primal, cold.primal) and np.array_equal(b.dual, cold.dual)
     for s in batch[1:]:
-        # the others may stall at one workgroup too; a restart shows as a
-        # capped warm pass
         assert s.status == "Optimal"
-        assert s.mixed["fallback_iterations"] == 0 or s.mixed["refinement_iterations"] == 100_000
 
 
 def test_uneven_sweep_batch_against_highs(hip32):
@@ -270,7 +271,9 @@ def test_uneven_sweep_batch_against_highs(hip32):
     probs = [prima.config3(d["args"]["k"]) for d in gold]
     for p, d in zip(probs, gold):
         assert (p.n, p.m, p.nnz) == (d["n"], d["m"], d["nnz"])
-    sols = hip32.solve_all(probs, prima.PdhgParams(max_iterations=1_000_000),
+    # a cold float64 pass of the median instance can need more than 700k
+    # iterations, depending on the trajectory (workgroups, twin timing)
+    sols = hip32.solve_all(probs, prima.PdhgParams(max_iterations=2_000_000),
                            mixed=prima.MixedParams(stall_iterations=300_000))
     for d, s in zip(gold, sols):
         assert s.status == "Optimal", (d["args"], s.status, s.mixed)
