@@ -355,6 +355,9 @@ def main():
     ap.add_argument("--unbatched", type=int, default=4,
                     help="scenarios solved one at a time for the FP64 denominator")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--fp64-batched", action="store_true",
+                    help="also solve the whole batch in plain FP64 (separates the batching "
+                         "gain from the precision gain; about as long as the mixed step)")
     ap.add_argument("--no-spmv", action="store_true")
     ap.add_argument("--only-spmv", action="store_true")
     ap.add_argument("--workload", default="dispatch",
@@ -428,6 +431,18 @@ def main():
 
     # plain FP64 PDHG, one LP at a time (the denominator), on the first
     # --unbatched scenarios of the same batch
+    fp64_batched = None
+    if args.fp64_batched:
+        k64.reset_stats()
+        t = time.perf_counter()
+        sb = run_watched(k64, lambda: k64.solve_all(batch, params), "fp64 batched")
+        tb = time.perf_counter() - t
+        check(sb, range(B), gold, "fp64 batched")
+        fp64_batched = {"value": B / tb, "seconds": tb,
+                        "iterations_mean": float(np.mean([s_.iterations for s_ in sb])),
+                        "iterations_max": int(max(s_.iterations for s_ in sb))}
+        log(f"fp64 batched {B / tb:.3f} LPs/s ({tb:.1f} s)")
+
     U = min(args.unbatched, B)
     k64.reset_stats()
     t = time.perf_counter()
@@ -466,6 +481,7 @@ def main():
         },
         "device_lps": B * args.steps / (st["kernel_ms"] * 1e-3) if st["kernel_ms"] else None,
         "fp64_unbatched_lps": fp64_lps,
+        "fp64_batched": fp64_batched,
         "fp64_unbatched_sample": U,
         "speedup_vs_fp64_unbatched": value / fp64_lps,
         "fp64_unbatched_iterations": [s.iterations for s in s64],
