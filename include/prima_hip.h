@@ -359,7 +359,9 @@ int prima_cancel(prima_ctx* ctx);
 
 /* Progress of the solves on ctx since it was opened, updated at every launch
  * boundary: solver launches, accepted PDHG iterations summed over LPs, and
- * LPs finished. No counterpart in the reference (its solves report nothing
+ * passes finished (a mixed-precision LP counts its float32 and float64
+ * passes, and a cold pass that raced the warm one). No counterpart in the
+ * reference (its solves report nothing
  * until they return); for callers that watch a long batch. Safe to call from
  * another thread while a solve runs. */
 int prima_progress(const prima_ctx* ctx, int64_t* launches, int64_t* accepted_iterations,
