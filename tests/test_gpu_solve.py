@@ -33,6 +33,28 @@ def gold(pred):
         return [d for d in json.load(f) if pred(d["args"])]
 
 
+def assert_kkt(p, s, eps=1e-8):
+    """The returned point itself meets Kkt.convergence (Kkt.scala:130-143) on
+    the original data: the oracle's Kkt.evaluate (Kkt.scala:68-125) recomputes
+    the residuals and both objectives from the primal and dual vectors alone
+    (fresh K x and K^T y in index order; the device's own evaluation differs
+    only in summation order, hence the 1e-6 relative slack). The primal lies
+    in its box, the inequality rows' duals in their cone, and the reported
+    reduced costs are c - K^T y."""
+    e = oracle_of(p).kkt(s.primal, s.dual)
+    slack = 1.0 + 1e-6
+    assert np.all(np.isfinite(e)), e
+    assert e[0] <= slack * (eps + eps * np.linalg.norm(p.q)), ("primal residual", e)
+    assert e[1] <= slack * (eps + eps * np.linalg.norm(p.c)), ("dual residual", e)
+    assert abs(e[2] - e[3]) <= slack * (eps + eps * (abs(e[2]) + abs(e[3]))), ("gap", e)
+    assert abs(e[2] - s.objective_value) <= 1e-9 * max(1.0, abs(e[2]))
+    room = 1e-12 * (1.0 + np.abs(s.primal))
+    assert np.all(s.primal >= p.l - room) and np.all(s.primal <= p.u + room)
+    assert np.all(s.dual[p.n_eq:] >= 0.0)
+    rc = p.c - p.to_scipy().T @ s.dual
+    assert np.allclose(s.reduced_costs, rc, rtol=1e-9, atol=1e-9 * (1.0 + np.abs(p.c).max()))
+
+
 @pytest.mark.parametrize("name", list(FIXTURES))
 def test_fixtures_fp64(hip64, name):
     # PdhgSuite.scala:12-35 at eps 1e-9, plus the ladder's counts
@@ -129,6 +151,7 @@ def test_random_ladder_against_oracle_and_published(hip64):
         s = hip64.solve(p, prima.PdhgParams(eps_abs=1e-9, eps_rel=1e-9, max_iterations=500_000))
         assert s.status == "Optimal"
         assert gap(s.objective_value, oj) <= 1e-8, (name, s.objective_value)
+        assert_kkt(p, s, 1e-9)
         assert s.restarts > 0
 
 
@@ -138,6 +161,7 @@ def test_config0_closed_form_fp64(hip64):
         s = hip64.solve(p)
         assert s.status == "Optimal"
         assert gap(s.objective_value, p.known_objective) <= 1e-8
+        assert_kkt(p, s)
 
 
 def test_dispatch_against_highs_fp64(hip64):
@@ -146,6 +170,7 @@ def test_dispatch_against_highs_fp64(hip64):
         s = hip64.solve(p, prima.PdhgParams(max_iterations=500_000))
         assert s.status == "Optimal"
         assert gap(s.objective_value, d["objective"]) <= 1e-8
+        assert_kkt(p, s)
 
 
 def test_mixed_precision_f32_device(hip32):
@@ -158,6 +183,7 @@ def test_mixed_precision_f32_device(hip32):
     p = prima.config0(1)
     s = hip32.mixed_solve(p, prima.PdhgParams())
     assert s.status == "Optimal" and gap(s.objective_value, p.known_objective) <= 1e-8
+    assert_kkt(p, s)
 
 
 def test_mixed_precision_fp64_device_skips_reduced_pass(hip64):
@@ -219,6 +245,7 @@ def test_multi_workgroup_groups_agree(hip64, wg):
     s = hip64.solve(p, prima.PdhgParams(max_iterations=500_000, workgroups_per_lp=wg))
     assert s.status == "Optimal" and s.workgroups == wg
     assert gap(s.objective_value, d["objective"]) <= 1e-8
+    assert_kkt(p, s)
 
 
 def test_mixed_stall_fallback(hip32):
@@ -257,8 +284,9 @@ def test_mixed_stall_fallback_in_a_batch(hip32, hip64):
     assert b.iterations == alone.iterations == cold.iterations
     assert np.array_equal(b.primal, alone.primal) and np.array_equal(b.dual, alone.dual)
     assert np.array_equal(b.primal, cold.primal) and np.array_equal(b.dual, cold.dual)
-    for s in batch[1:]:
+    for p, s in zip(probs, batch):
         assert s.status == "Optimal"
+        assert_kkt(p, s)
 
 
 def test_uneven_sweep_batch_against_highs(hip32):
@@ -275,9 +303,10 @@ def test_uneven_sweep_batch_against_highs(hip32):
     # iterations, depending on the trajectory (workgroups, twin timing)
     sols = hip32.solve_all(probs, prima.PdhgParams(max_iterations=2_000_000),
                            mixed=prima.MixedParams(stall_iterations=300_000))
-    for d, s in zip(gold, sols):
+    for p, d, s in zip(probs, gold, sols):
         assert s.status == "Optimal", (d["args"], s.status, s.mixed)
         assert gap(s.objective_value, d["objective"]) <= 1e-8, (d["args"], s.objective_value)
+        assert_kkt(p, s)
 
 
 def test_batch_beyond_capacity_admits_waiting_lps(hip64):
