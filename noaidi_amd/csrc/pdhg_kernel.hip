@@ -739,11 +739,12 @@ __global__ __launch_bounds__(NT) void pdhg_persistent(const LpView* __restrict__
   __shared__ Smem sm;
   T* tile = reinterpret_cast<T*>(sm.tile);
 
-  const LpView& v = views[wg_lp[blockIdx.x]];
+  const int32_t me = wg_lp[blockIdx.x];  // LP index << 9 | rank in its group
+  const LpView& v = views[me >> 9];
   const Lp<T> L(v);
   Ranges R;
   R.G = v.G;
-  R.rank = (int)blockIdx.x - v.wg_base;
+  R.rank = me & 511;
   R.tb0 = v.t_wrb[R.rank];
   R.tb1 = v.t_wrb[R.rank + 1];
   R.ab0 = v.a_wrb[R.rank];

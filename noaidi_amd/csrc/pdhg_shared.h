@@ -40,8 +40,7 @@ struct KParams {
 // pointers; T-typed arrays are stored as void* and reinterpreted.
 struct LpView {
   int32_t n, m, n_eq;
-  int32_t G;        // workgroups in this group
-  int32_t wg_base;  // first blockIdx.x of this group
+  int32_t G;         // workgroups in this group
   int32_t max_iter;  // this LP's iteration cap
   const int32_t *a_ptr, *a_col;
   const void* a_val;
