@@ -340,7 +340,8 @@ def uc_frontier(args):
     # the headline's caps and mixed-precision knobs: node iteration counts
     # have the same heavy tail
     P = prima.PdhgParams(max_iterations=4_000_000, workgroups_per_lp=args.wg)
-    mixed = prima.MixedParams(reduced_tolerance=1e-6, reduced_max_iterations=60_000,
+    mixed = prima.MixedParams(reduced_tolerance=args.reduced_tolerance,
+                              reduced_max_iterations=args.reduced_max_iterations,
                               stall_iterations=1_000_000)
     k64 = prima.HipKernels(0, 64)
     k32 = prima.HipKernels(0, 32)
@@ -399,6 +400,10 @@ def main():
                     help="also solve the whole batch in plain FP64 (separates the batching "
                          "gain from the precision gain; about as long as the mixed step)")
     ap.add_argument("--no-spmv", action="store_true")
+    ap.add_argument("--reduced-tolerance", type=float, default=1e-6,
+                    help="tolerance of the float32 pass (MixedPrecision.Params.reducedTolerance)")
+    ap.add_argument("--reduced-max-iterations", type=int, default=60_000,
+                    help="iteration cap of the float32 pass")
     ap.add_argument("--only-spmv", action="store_true")
     ap.add_argument("--workload", default="dispatch",
                     choices=["dispatch", "uc-frontier", "kernel", "lopf"])
@@ -440,7 +445,8 @@ def main():
     # the float64 pass a closer start and fewer stalls (256 scenarios: 84.6 s
     # against 118.7 s at 1e-5 / 60k); a warm-started float64 pass that has not
     # converged after 1M iterations restarts cold (stall_iterations)
-    mixed = prima.MixedParams(reduced_tolerance=1e-6, reduced_max_iterations=60_000,
+    mixed = prima.MixedParams(reduced_tolerance=args.reduced_tolerance,
+                              reduced_max_iterations=args.reduced_max_iterations,
                               stall_iterations=1_000_000)
     k32 = prima.HipKernels(0, 32)
     k64 = prima.HipKernels(0, 64)
