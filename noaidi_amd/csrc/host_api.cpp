@@ -2280,7 +2280,7 @@ int prima_vec_free(prima_ctx* ctx, prima_vec* v) {
   return PRIMA_OK;
 }
 
-// Column-panel layout of a CSR matrix (PRIMA_SPMV_PANELS). Panels are 1 MiB
+// Column-panel layout of a CSR matrix (PRIMA_SPMV_PANELS). Panels are 512 KiB
 // of x wide; rows are grouped greedily into ranges of at most panel_rows()
 // rows whose tiles each hold at most panel_tile() entries; a range's entries
 // are stored panel by panel, each tile row by row in the row's stored order.
