@@ -2494,16 +2494,20 @@ int prima_mat_autotune(prima_ctx* ctx, prima_mat* a, int32_t reps, int32_t* chos
   if (rc) return rc;
   if (!a || reps <= 0) return fail(ctx, PRIMA_EINVAL, "bad autotune arguments");
   const size_t s = esize(ctx->precision);
-  void *x = nullptr, *y = nullptr;
-  HIP_TRY(ctx, hipMalloc(&x, std::max<size_t>(1, (size_t)a->cols * s)));
-  HIP_TRY(ctx, hipMalloc(&y, std::max<size_t>(1, (size_t)a->rows * s)));
-  struct Free {
-    void *x, *y;
-    ~Free() {
-      (void)hipFree(x);
-      (void)hipFree(y);
+  // owns both buffers from before the first allocation, so an early return
+  // frees whichever exists
+  struct Buffers {
+    void* x = nullptr;
+    void* y = nullptr;
+    ~Buffers() {
+      if (x) (void)hipFree(x);
+      if (y) (void)hipFree(y);
     }
-  } guard{x, y};
+  } buf;
+  HIP_TRY(ctx, hipMalloc(&buf.x, std::max<size_t>(1, (size_t)a->cols * s)));
+  HIP_TRY(ctx, hipMalloc(&buf.y, std::max<size_t>(1, (size_t)a->rows * s)));
+  void* const x = buf.x;
+  void* const y = buf.y;
   HIP_TRY(ctx, hipMemsetAsync(x, 0, std::max<size_t>(1, (size_t)a->cols * s), ctx->stream));
   double best = INFINITY;
   int pick = PRIMA_SPMV_ROW_BLOCKS;
