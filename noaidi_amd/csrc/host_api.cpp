@@ -2241,13 +2241,22 @@ int prima_vec_alloc(prima_ctx* ctx, int64_t n, prima_vec** out) {
 }
 
 int prima_vec_upload(prima_ctx* ctx, const double* data, int64_t n, prima_vec** out) {
-  int rc = prima_vec_alloc(ctx, n, out);
+  int rc = check_open(ctx);
   if (rc) return rc;
-  if (n == 0) return PRIMA_OK;
-  if (!data) return fail(ctx, PRIMA_EINVAL, "null data");
-  std::vector<char> tmp((size_t)n * esize(ctx->precision));
-  to_prec(ctx->precision, data, n, tmp.data());
-  HIP_TRY(ctx, hipMemcpy((*out)->d, tmp.data(), tmp.size(), hipMemcpyHostToDevice));
+  if (n > 0 && !data) return fail(ctx, PRIMA_EINVAL, "null data");
+  prima_vec* v = nullptr;
+  rc = prima_vec_alloc(ctx, n, &v);
+  if (rc) return rc;
+  if (n > 0) {
+    std::vector<char> tmp((size_t)n * esize(ctx->precision));
+    to_prec(ctx->precision, data, n, tmp.data());
+    const hipError_t e = hipMemcpy(v->d, tmp.data(), tmp.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      (void)prima_vec_free(ctx, v);
+      return dev_fail(ctx, e, "vector upload");
+    }
+  }
+  *out = v;
   return PRIMA_OK;
 }
 
@@ -2256,6 +2265,7 @@ int prima_vec_download(prima_ctx* ctx, const prima_vec* v, double* into, int64_t
   if (rc) return rc;
   if (!v || n != v->n) return fail(ctx, PRIMA_EINVAL, "length mismatch");
   if (n == 0) return PRIMA_OK;
+  if (!into) return fail(ctx, PRIMA_EINVAL, "null destination");
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   if (ctx->precision == 64) {
     HIP_TRY(ctx, hipMemcpy(into, v->d, (size_t)n * 8, hipMemcpyDeviceToHost));
