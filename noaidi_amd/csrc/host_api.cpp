@@ -654,8 +654,9 @@ inline int32_t wg_entry(size_t k, int rank) { return (int32_t)(((int64_t)k << 9)
 // XCDs round-robin (on MI355X's 8: workgroup w to XCD w mod 8), and each XCD
 // has its own L2. The table is filled LP by LP; this reorders it so that
 // consecutive entries take the slots of one XCD before the next (XCD 0 gets
-// workgroups 0, 8, 16, ...), so a group of up to 32 workgroups sits on one
-// XCD. Placement changes no arithmetic (results are bit-identical); on the
+// workgroups 0, 8, 16, ...), so a group sits on as few XCDs as its size and
+// the launch's width allow (on a 256-workgroup launch, a group of up to 32 on
+// one XCD). Placement changes no arithmetic (results are bit-identical); on the
 // 65k-nnz dispatch LP, f64 microseconds per iteration went from 92.8 to 79.6
 // for 32 LPs at G = 8, 138 to 126 for 64 at G = 4, 41.6 to 40.1 for 8 at
 // G = 32 and 21.9 to 20.8 for one LP at G = 64.
