@@ -367,6 +367,16 @@ int prima_cancel(prima_ctx* ctx);
 int prima_progress(const prima_ctx* ctx, int64_t* launches, int64_t* accepted_iterations,
                    int64_t* finished_lps);
 
+/* Test hook with no counterpart in the reference. While enabled, every solve
+ * on ctx whose LPs run on more than one workgroup has the last workgroup of
+ * each group wait, with a short limit, for a word nothing writes: its bounded
+ * wait gives up and sets the group's abort word, the group's other
+ * workgroups give up at their next barrier, and every workgroup exits. The
+ * solve call then fails with PRIMA_EDEVICE, as it does when a group barrier
+ * times out for real (a group barrier giving up is a device failure, never a
+ * SolveStatus). For tests of that error path; 0 turns it off. */
+int prima_test_stall_barriers(prima_ctx* ctx, int32_t enable);
+
 #ifdef __cplusplus
 }
 #endif

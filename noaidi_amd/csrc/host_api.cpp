@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/prima_hip.h"
@@ -21,7 +22,8 @@
 namespace prima {
 // pdhg_kernel.hip
 int occupancy_blocks_per_cu(int precision);
-hipError_t launch_pdhg(int precision, const LpView* views, const int32_t* wg_lp, int grid,
+hipError_t launch_pdhg(int precision, const LpView* views, const int32_t* wg_off,
+                       const int32_t* wg_ent, int grid,
                        const KParams& P, hipStream_t stream);
 int threads_per_block();
 int tile_nnz();
@@ -81,6 +83,7 @@ struct prima_ctx {
   int occ[2] = {0, 0};  // blocks per CU for float / double kernels
   double* scratch = nullptr;  // reduction partials for dot / squaredNorm
   std::atomic<int> cancel{0};
+  int test_stall = 0;  // test hook: every group's barrier wait gives up
   // progress of the running solve, readable from another thread
   std::atomic<int64_t> prog_launches{0}, prog_iterations{0}, prog_finished{0};
   int64_t prog_base = 0;  // accepted iterations of finished waves
@@ -535,10 +538,12 @@ struct Wave {
   std::vector<int> idx;  // indices into the batch
   std::vector<LpLayout> lay;
   std::vector<LpView> views;
-  std::vector<int32_t> wg_lp;
+  // launch plan: workgroup w runs the chunks ent[off[w] .. off[w+1])
+  std::vector<int32_t> off, ent;
   char* arena = nullptr;
   LpView* d_views = nullptr;
-  int32_t* d_wg_lp = nullptr;
+  int32_t* d_off = nullptr;
+  int32_t* d_ent = nullptr;
   int grid = 0;
 };
 
@@ -616,38 +621,102 @@ struct SolveJob {
   int shadow = -1, shadow_of = -1;
 };
 
-// Workgroups per LP. Alone, an LP is latency-bound: spread it at about 1k
-// nonzeros per workgroup. In a batch, filling the device pays (65k-nnz
-// dispatch LP, f64 LP-iterations/s with all 256 workgroups against half of
-// them: 128 LPs 522k at G = 2 vs 395k at 1; 64 LPs 451k at 4 vs 381k at 2;
-// 32 LPs 343k at 8 vs 331k at 4; 16 LPs 295k at 16 vs 237k at 8), except
-// that four or more LPs gain nothing beyond 32 workgroups each (4 LPs: 30.8
-// us per iteration at 32, 32.8 at 64). So each LP gets its share of the
-// device, capped by its alone size and, from four LPs on, by 32. G_cap, which
-// sizes an LP's per-rank arrays and bounds later regrouping, is the alone
-// size, so a batch's last LPs can spread out as the others finish.
-void choose_groups(const prima_params* p, std::vector<SolveJob>& jobs, int capacity) {
-  if (p->workgroups_per_lp > 0) {
-    for (auto& J : jobs) {
-      J.h.G = std::min(p->workgroups_per_lp, capacity);
-      J.h.G_cap = std::max(J.h.G_cap, J.h.G);
+// Compulsory bytes of one accepted iteration (each array counted once, no
+// gather amplification): 2Z(s+4) + 4(m+n+2) + s(13n+11m).
+double iter_bytes(const HostLp& h, int prec) {
+  const double Z = (double)h.nnz, n = h.n, m = h.m, sb = (double)esize(prec);
+  return 2 * Z * (sb + 4) + 4 * (m + n + 2) + sb * (13 * n + 11 * m);
+}
+
+// Cost model of one accepted iteration on G workgroups, in microseconds: a
+// workgroup's share of HBM with the device full (about 16 GB/s: 4 TB/s over
+// 256 workgroups, as measured on batches of dispatch LPs), two group barriers
+// of about 4.5 us when G > 1, and about 8 us of dependent memory round trips
+// per iteration at any size (one dispatch LP alone on 64 workgroups: 20.8 us
+// measured, 21.4 modelled; 256 at one workgroup each: 415 us, 423 modelled).
+// Deterministic on purpose: launch lengths, and with them the iterations at
+// which LPs are regrouped, do not depend on timing.
+double model_iter_us(const HostLp& h, int prec, int G) {
+  constexpr double kWorkgroupBytesPerUs = 16000.0;
+  return iter_bytes(h, prec) / (G * kWorkgroupBytesPerUs) + (G > 1 ? 9.0 : 0.0) + 8.0;
+}
+
+// Most workgroups one LP's group may have: ranks are packed into 9 bits of a
+// launch-table entry.
+constexpr int kMaxGroup = 512;
+
+// Workgroups per LP for a set of LPs solved together. Alone an LP is latency
+// bound and spreads at about 1k nonzeros per workgroup (G_cap, which sizes
+// its per-rank arrays and bounds any later regrouping). Together, each LP
+// gets a share of the device in proportion to its bytes per iteration, so
+// that every LP iterates at about the same rate: an LP whose share rounds to
+// fewer than two workgroups runs alone on one, packed with others onto the
+// workgroups the groups leave free (enough for those LPs' bytes). From four
+// LPs on a group is capped at 32 workgroups unless it moves more than 64 MB
+// per iteration (at 65k nonzeros four LPs ran 30.8 us per iteration at 32
+// workgroups each, 32.8 at 64). Twins (shadow_of >= 0) do not count towards
+// the shares until they run.
+void choose_groups(const prima_params* p, const std::vector<SolveJob*>& js, int capacity,
+                   int prec) {
+  const int gmax = std::max(1, std::min(capacity, kMaxGroup));
+  for (SolveJob* J : js) {
+    const int64_t nnz = std::max<int64_t>(J->h.nnz, 1);
+    const int alone = (int)std::max<int64_t>(1, std::min<int64_t>((nnz + 1023) / 1024, gmax));
+    J->h.G_cap = std::max(J->h.G_cap, alone);
+    if (p->workgroups_per_lp > 0) {
+      J->h.G = std::min(p->workgroups_per_lp, gmax);
+      J->h.G_cap = std::max(J->h.G_cap, J->h.G);
     }
-    return;
   }
-  int64_t B = 0;  // cold twins (run_wave) do not count: they only race a warm pass
-  for (const auto& J : jobs) B += J.shadow_of < 0;
-  const int64_t share = std::max<int64_t>(1, capacity / std::max<int64_t>(B, 1));
-  for (auto& J : jobs) {
-    const int64_t nnz = std::max<int64_t>(J.h.nnz, 1);
-    const int64_t alone = std::max<int64_t>(1, std::min<int64_t>((nnz + 1023) / 1024, capacity));
-    J.h.G_cap = std::max(J.h.G_cap, (int)alone);
-    int64_t g = std::min(alone, share);
-    if (B >= 4) g = std::min<int64_t>(g, 32);
-    J.h.G = (int)std::max<int64_t>(1, g);
+  if (p->workgroups_per_lp > 0) return;
+  double total = 0;
+  int64_t B = 0;
+  for (SolveJob* J : js)
+    if (J->shadow_of < 0) {
+      total += iter_bytes(J->h, prec);
+      ++B;
+    }
+  const double per_wg = total > 0 ? total / capacity : 1.0;
+  for (SolveJob* J : js) {
+    const double bytes = iter_bytes(J->h, prec);
+    const double r = bytes / per_wg;
+    int64_t g = r >= 1.5 ? (int64_t)std::llround(r) : 1;
+    if (B >= 4) g = std::min<int64_t>(g, std::max<int64_t>(32, (int64_t)(bytes / 2.0e6)));
+    J->h.G = (int)std::max<int64_t>(1, std::min<int64_t>(g, J->h.G_cap));
+  }
+  // leave the single-workgroup LPs enough workgroups for their bytes (at
+  // least one): shrink the group with the lightest load per workgroup first
+  for (;;) {
+    int64_t grouped = 0;
+    double single_bytes = 0;
+    bool any_single = false;
+    for (SolveJob* J : js) {
+      if (J->shadow_of >= 0) continue;
+      if (J->h.G > 1) grouped += J->h.G;
+      else {
+        single_bytes += iter_bytes(J->h, prec);
+        any_single = true;
+      }
+    }
+    const int64_t need =
+        any_single ? std::max<int64_t>(1, (int64_t)std::ceil(single_bytes / per_wg - 1e-9)) : 0;
+    if (grouped + need <= capacity) break;
+    SolveJob* lightest = nullptr;
+    double lw = INFINITY;
+    for (SolveJob* J : js) {
+      if (J->shadow_of >= 0 || J->h.G <= 1) continue;
+      const double w = iter_bytes(J->h, prec) / J->h.G;
+      if (w < lw) {
+        lw = w;
+        lightest = J;
+      }
+    }
+    if (!lightest) break;
+    lightest->h.G -= 1;
   }
 }
 
-// Launch table entries: LP index << 9 | rank in its group (G <= 256).
+// Launch table entries: LP index << 9 | rank in its group (G <= kMaxGroup).
 inline int32_t wg_entry(size_t k, int rank) { return (int32_t)(((int64_t)k << 9) | rank); }
 
 // Workgroup placement. The dispatcher hands a launch's workgroups to the
@@ -659,15 +728,15 @@ inline int32_t wg_entry(size_t k, int rank) { return (int32_t)(((int64_t)k << 9)
 // one XCD). Placement changes no arithmetic (results are bit-identical); on the
 // 65k-nnz dispatch LP, f64 microseconds per iteration went from 92.8 to 79.6
 // for 32 LPs at G = 8, 138 to 126 for 64 at G = 4, 41.6 to 40.1 for 8 at
-// G = 32 and 21.9 to 20.8 for one LP at G = 64.
-void place_by_xcd(std::vector<int32_t>& t, int xcds) {
-  if (xcds <= 1) return;
-  const size_t n = t.size();
-  std::vector<int32_t> out(n);
-  size_t i = 0;
-  for (size_t x = 0; x < (size_t)xcds; ++x)
-    for (size_t w = x; w < n; w += (size_t)xcds) out[w] = t[i++];
-  t.swap(out);
+// G = 32 and 21.9 to 20.8 for one LP at G = 64. Returns the workgroup index
+// of each of n consecutive slots.
+std::vector<int32_t> xcd_order(size_t n, int xcds) {
+  std::vector<int32_t> out;
+  out.reserve(n);
+  const size_t x_n = (size_t)std::max(1, xcds);
+  for (size_t x = 0; x < x_n; ++x)
+    for (size_t w = x; w < n; w += x_n) out.push_back((int32_t)w);
+  return out;
 }
 
 // Upload one wave, run it to completion in chunks, download the results.
@@ -691,8 +760,10 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
   }
   // one contiguous block of per-LP states, read back with a single copy at
   // every launch boundary
-  const size_t st_base = cv.take(idx.size() * sizeof(SolveState));
+  const size_t st_base = cv.take(idx.size() * sizeof(SolveState) + 8);
   for (size_t k = 0; k < idx.size(); ++k) W.lay[k].st = st_base + k * sizeof(SolveState);
+  // the launch-wide abort word follows the states (read back with them)
+  const size_t abort_off = st_base + idx.size() * sizeof(SolveState);
   const size_t arena_bytes = cv.off;
   auto t_up = clk::now();
   void* arena = nullptr;
@@ -705,7 +776,8 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
     ~Guard() {
       if (w.arena) (void)hipFree(w.arena);
       if (w.d_views) (void)hipFree(w.d_views);
-      if (w.d_wg_lp) (void)hipFree(w.d_wg_lp);
+      if (w.d_off) (void)hipFree(w.d_off);
+      if (w.d_ent) (void)hipFree(w.d_ent);
     }
   } guard{W};
   HIP_TRY(ctx, hipMemsetAsync(arena, 0, arena_bytes, ctx->stream));
@@ -779,11 +851,11 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
     return e;
   };
 
-  // Continuous batching: LPs are admitted in input order while their
-  // workgroups fit the co-resident capacity; the rest wait and are admitted
-  // into the workgroups that finishing LPs free (see regroup below).
+  // Every LP of the wave is admitted at once: an LP on more than one
+  // workgroup holds them for a launch, and LPs on one workgroup are packed,
+  // several to a workgroup if need be (see plan below).
   const size_t nlp = idx.size();
-  std::vector<char> active(nlp, 0);
+  std::vector<char> active(nlp, 1);
   // cold twins of warm passes wait dormant until they race (see below);
   // dropped slots ran but do not report
   std::vector<char> dormant(nlp, 0), dropped(nlp, 0);
@@ -792,18 +864,6 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
     slot_of[idx[k]] = (int)k;
     dormant[k] = jobs[idx[k]].shadow_of >= 0;
   }
-  {
-    int used = 0;
-    for (size_t k = 0; k < nlp; ++k) {
-      if (dormant[k]) continue;
-      const int g = jobs[idx[k]].h.G;
-      if (used + g > capacity) break;
-      active[k] = 1;
-      used += g;
-    }
-  }
-  int wg_base = 0;
-  W.wg_lp.clear();
   for (size_t k = 0; k < idx.size(); ++k) {
     SolveJob& J = jobs[idx[k]];
     HostLp& h = J.h;
@@ -901,6 +961,7 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
     v.part = (double*)(a + L.part);
     v.bar = (uint32_t*)(a + L.bar);
     v.st = (SolveState*)(a + L.st);
+    v.launch_abort = (uint32_t*)(a + abort_off);
     v.ylo = J.ylo ? (const void*)(a + L.ylo) : nullptr;
     v.oylo = J.ylo ? (const double*)(a + L.oylo) : nullptr;
     v.offset = lp->offset;
@@ -908,20 +969,11 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
     v.obj_norm = h.obj_norm;
     v.spec_orig = h.M->spec_orig;
     W.views.push_back(v);
-    if (active[k]) {
-      for (int g = 0; g < h.G; ++g) W.wg_lp.push_back(wg_entry(k, g));
-      wg_base += h.G;
-    }
   }
-  W.grid = wg_base;
-  place_by_xcd(W.wg_lp, ctx->xcds);
   HIP_TRY(ctx, hipMalloc((void**)&W.d_views, W.views.size() * sizeof(LpView)));
-  HIP_TRY(ctx, hipMalloc((void**)&W.d_wg_lp,
-                         std::max<size_t>(W.wg_lp.size(), (size_t)capacity) * sizeof(int32_t)));
-  HIP_TRY(ctx, hipMemcpy(W.d_views, W.views.data(), W.views.size() * sizeof(LpView),
-                         hipMemcpyHostToDevice));
-  HIP_TRY(ctx, hipMemcpy(W.d_wg_lp, W.wg_lp.data(), W.wg_lp.size() * sizeof(int32_t),
-                         hipMemcpyHostToDevice));
+  // a plan has at most `capacity` workgroups; each LP appears G times
+  HIP_TRY(ctx, hipMalloc((void**)&W.d_off, ((size_t)capacity + 1) * sizeof(int32_t)));
+  HIP_TRY(ctx, hipMalloc((void**)&W.d_ent, ((size_t)capacity + nlp) * sizeof(int32_t)));
   const double upload_ms = ms_since(t_up);
 
   // chunked launches until every LP of the wave reached a status
@@ -937,117 +989,190 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
   P.eval_freq = p->evaluation_frequency;
   P.mode = 0;
   P.certificates = certificates ? 1 : 0;
+  P.test_stall = ctx->test_stall;
   std::vector<SolveState> states(nlp);
-  for (auto& st : states) st.status = -1;
-  // Iterations per launch. Without a time limit the chunk length follows a
-  // cost model of the LPs still running (about 50 ms at 3 TB/s, never under
-  // 30 us per iteration), so launch boundaries -- and with them the points
-  // where finished LPs hand their workgroups over -- fall on the same
-  // iterations every run and batched results are reproducible. Under a time
-  // limit the length follows the measured launch time instead.
-  const bool timed_chunks = p->chunk_iterations == 0 && p->time_limit_ms >= 0;
-  auto model_budget = [&]() {
-    double bytes = 0;
-    for (size_t k = 0; k < nlp; ++k) {
-      if (states[k].status >= 0 || !active[k]) continue;
-      const HostLp& h = jobs[idx[k]].h;
-      const double Z = (double)h.nnz, n = h.n, m = h.m, sb = (double)esize(prec);
-      bytes += 2 * Z * (sb + 4) + 4 * (m + n + 2) + sb * (13 * n + 11 * m);
-    }
-    const double us = std::max(30.0, bytes / 3.0e6);
-    const int64_t f = std::max(1, p->evaluation_frequency);
-    int64_t b = (int64_t)(50000.0 / us);
-    b = std::max<int64_t>(f, std::min<int64_t>(1000000, (b + f - 1) / f * f));
-    return (int)b;
+  // A group barrier that gave up (the kernel's bounded wait) is a device
+  // failure, not a solver outcome: the whole call fails with PRIMA_EDEVICE.
+  auto check_abort = [&]() -> int {
+    uint32_t word = 0;
+    HIP_TRY(ctx, hipMemcpy(&word, W.arena + abort_off, sizeof(word), hipMemcpyDeviceToHost));
+    if (word == 0) return PRIMA_OK;
+    const size_t k = (size_t)word - 1;
+    const int G = k < nlp ? W.views[k].G : 0;
+    return fail(ctx, PRIMA_EDEVICE,
+                "a group barrier of the solver kernel gave up (LP " +
+                    std::to_string(k < nlp ? idx[k] : -1) + ", " + std::to_string(G) +
+                    " workgroups): a workgroup of the group did not arrive in time");
   };
-  int budget = p->chunk_iterations > 0 ? p->chunk_iterations : (timed_chunks ? 1024 : model_budget());
+  for (auto& st : states) st.status = -1;
+  // Launch plan. Each running LP on G > 1 workgroups holds G workgroups
+  // for the launch; the LPs on one workgroup are packed, longest modelled
+  // iteration first onto the least loaded, onto the workgroups the groups
+  // leave free. Each LP's chunk (iterations before it stops at a checkpoint)
+  // follows the cost model so that every workgroup's list takes about the
+  // same time, about 50 ms, and at least one evaluation period. Without a
+  // time limit the plan depends on the model alone, so launch boundaries --
+  // and with them the points where LPs are regrouped -- fall on the same
+  // iterations every run and batched results are reproducible. Under a time
+  // limit the target shrinks with the limit and follows the measured launch
+  // times. chunk_iterations > 0 fixes every chunk instead.
+  const bool timed_chunks = p->chunk_iterations == 0 && p->time_limit_ms >= 0;
+  const int64_t freq = std::max(1, p->evaluation_frequency);
+  double model_scale = 1.0;    // measured / modelled launch time (time limits only)
+  double planned_us = 0.0;     // modelled length of the launch just planned
+  auto running_lp = [&](size_t k) {
+    return active[k] && !dormant[k] && !dropped[k] && states[k].status < 0;
+  };
+  auto plan = [&](double target_us) -> int {
+    std::vector<size_t> groups, singles;
+    for (size_t k = 0; k < nlp; ++k) {
+      if (!running_lp(k)) continue;
+      (jobs[idx[k]].h.G > 1 ? groups : singles).push_back(k);
+    }
+    std::vector<std::vector<int32_t>> bins;
+    std::vector<double> bin_us;
+    for (size_t k : groups) {
+      const HostLp& h = jobs[idx[k]].h;
+      const double t = model_iter_us(h, prec, h.G);
+      for (int g = 0; g < h.G; ++g) {
+        bins.push_back({wg_entry(k, g)});
+        bin_us.push_back(t);
+      }
+    }
+    const size_t first_single = bins.size();
+    if (bins.size() + (singles.empty() ? 0 : 1) > (size_t)capacity)
+      return fail(ctx, PRIMA_EDEVICE, "launch plan exceeds the co-resident workgroups (" +
+                                          std::to_string(bins.size()) + " grouped of " +
+                                          std::to_string(capacity) + ")");
+    if (!singles.empty()) {
+      const size_t room = bins.size() < (size_t)capacity ? (size_t)capacity - bins.size() : 1;
+      const size_t nb = std::min(singles.size(), std::max<size_t>(1, room));
+      std::vector<double> t1(nlp, 0.0);
+      for (size_t k : singles) t1[k] = model_iter_us(jobs[idx[k]].h, prec, 1);
+      std::stable_sort(singles.begin(), singles.end(),
+                       [&](size_t a, size_t b) { return t1[a] > t1[b]; });
+      bins.resize(first_single + nb);
+      bin_us.resize(first_single + nb, 0.0);
+      for (size_t k : singles) {
+        size_t best = first_single;
+        for (size_t b = first_single + 1; b < bins.size(); ++b)
+          if (bin_us[b] < bin_us[best]) best = b;
+        bins[best].push_back(wg_entry(k, 0));
+        bin_us[best] += t1[k];
+      }
+    }
+    // one launch lasts as long as its longest list; every list gets at least
+    // one evaluation period of iterations
+    double T = target_us;
+    for (double t : bin_us) T = std::max(T, (double)freq * t);
+    planned_us = 0.0;
+    for (size_t b = 0; b < bins.size(); ++b) {
+      int64_t budget = p->chunk_iterations > 0
+                           ? (int64_t)p->chunk_iterations
+                           : std::max<int64_t>(freq, (int64_t)(T / bin_us[b]) / freq * freq);
+      budget = std::min<int64_t>(budget, 100000000);
+      planned_us = std::max(planned_us, budget * bin_us[b]);
+      for (int32_t e : bins[b]) W.views[(size_t)(e >> 9)].iter_budget = (int32_t)budget;
+    }
+    // consecutive slots of one XCD first (see xcd_order)
+    const std::vector<int32_t> where = xcd_order(bins.size(), ctx->xcds);
+    std::vector<int32_t> slot_of_wg(bins.size());
+    for (size_t b = 0; b < bins.size(); ++b) slot_of_wg[(size_t)where[b]] = (int32_t)b;
+    W.off.assign(1, 0);
+    W.ent.clear();
+    for (size_t w = 0; w < bins.size(); ++w) {
+      for (int32_t e : bins[(size_t)slot_of_wg[w]]) W.ent.push_back(e);
+      W.off.push_back((int32_t)W.ent.size());
+    }
+    W.grid = (int)bins.size();
+    HIP_TRY(ctx, hipMemcpy(W.d_views, W.views.data(), W.views.size() * sizeof(LpView),
+                           hipMemcpyHostToDevice));
+    HIP_TRY(ctx, hipMemcpy(W.d_off, W.off.data(), W.off.size() * sizeof(int32_t),
+                           hipMemcpyHostToDevice));
+    if (!W.ent.empty())
+      HIP_TRY(ctx, hipMemcpy(W.d_ent, W.ent.data(), W.ent.size() * sizeof(int32_t),
+                             hipMemcpyHostToDevice));
+    return PRIMA_OK;
+  };
   std::vector<double> done_ms(nlp, -1.0), admit_ms(nlp, 0.0);
-  int running_groups = 0;
-  for (size_t k = 0; k < nlp; ++k) running_groups += active[k];
   auto t_solve = clk::now();
-  // Elastic batching: when LPs finish, the survivors are re-partitioned over
-  // the freed workgroups. Safe at a launch boundary, where every LP's state
-  // is committed; only the ownership tables and group sizes change.
+  // Elastic batching: when the set of running LPs changes (LPs finish, a
+  // twin starts), group sizes are chosen again for the survivors, and an LP
+  // whose size moved by half or more is re-partitioned (row blocks, rank
+  // ranges). Safe at a launch boundary, where every LP's state is committed.
+  std::vector<char> was_running(nlp, 0);
   auto regroup = [&]() -> int {
     std::vector<size_t> which;
-    std::vector<size_t> waiting;
+    bool changed = false;
     for (size_t k = 0; k < nlp; ++k) {
-      if (states[k].status >= 0 || dormant[k]) continue;
-      if (active[k]) which.push_back(k);
-      else waiting.push_back(k);
+      const bool r = running_lp(k);
+      changed |= r != (bool)was_running[k];
+      was_running[k] = r;
+      if (r) which.push_back(k);
     }
-    if (which.empty() && waiting.empty()) return PRIMA_OK;
-    std::vector<int> newG;
-    if (!waiting.empty()) {
-      // admit waiting LPs (input order, planned group size) into the freed
-      // workgroups; survivors keep their groups
-      int used = 0;
-      for (size_t k : which) used += jobs[idx[k]].h.G;
-      for (size_t k : which) newG.push_back(jobs[idx[k]].h.G);
-      for (size_t k : waiting) {
-        const int g = jobs[idx[k]].h.G;
-        if (used + g > capacity) break;
-        active[k] = 1;
-        admit_ms[k] = ms_since(t_solve);
-        used += g;
-        which.push_back(k);
-        newG.push_back(g);
-      }
-    } else {
-      newG.resize(which.size());
-      // no LP waits: the survivors share the device (the initial policy)
-      std::vector<SolveJob> tmp(which.size());
-      for (size_t j = 0; j < which.size(); ++j) {
-        tmp[j].h.nnz = jobs[idx[which[j]]].h.nnz;
-        tmp[j].h.G_cap = jobs[idx[which[j]]].h.G_cap;
-      }
-      choose_groups(p, tmp, capacity);
+    if (!changed || which.empty()) return PRIMA_OK;
+    std::vector<SolveJob> tmp(which.size());
+    std::vector<SolveJob*> ptrs(which.size());
+    for (size_t j = 0; j < which.size(); ++j) {
+      const HostLp& h = jobs[idx[which[j]]].h;
+      tmp[j].h.n = h.n;
+      tmp[j].h.m = h.m;
+      tmp[j].h.nnz = h.nnz;
+      tmp[j].h.G_cap = h.G_cap;
+      ptrs[j] = &tmp[j];
+    }
+    choose_groups(p, ptrs, capacity, prec);
+    // keep a group size that is within a factor of 1.5 of the new choice,
+    // unless the kept sizes would not leave the packed LPs a workgroup
+    std::vector<int> next(which.size());
+    int64_t grouped = 0;
+    bool any_single = false;
+    for (size_t j = 0; j < which.size(); ++j) {
+      const int want = std::min(tmp[j].h.G, jobs[idx[which[j]]].h.G_cap);
+      const int cur = jobs[idx[which[j]]].h.G;
+      next[j] = (2 * want < 3 * cur && 2 * cur < 3 * want) ? cur : want;
+      if (next[j] > 1) grouped += next[j];
+      else any_single = true;
+    }
+    if (grouped + (any_single ? 1 : 0) > capacity)
       for (size_t j = 0; j < which.size(); ++j)
-        newG[j] = std::min(tmp[j].h.G, jobs[idx[which[j]]].h.G_cap);
-    }
-    if (which.empty()) return PRIMA_OK;
-    bool changed = which.size() != (size_t)running_groups;
-    for (size_t j = 0; j < which.size(); ++j) changed |= newG[j] != jobs[idx[which[j]]].h.G;
-    if (!changed) return PRIMA_OK;
-    running_groups = (int)which.size();
-    W.wg_lp.clear();
-    int base = 0;
+        next[j] = std::min(tmp[j].h.G, jobs[idx[which[j]]].h.G_cap);
     for (size_t j = 0; j < which.size(); ++j) {
       const size_t k = which[j];
       SolveJob& J = jobs[idx[k]];
-      const bool regrouped = J.h.G != newG[j];
-      J.h.G = newG[j];
-      if (regrouped) {
-        static const int32_t zero_ptr[1] = {0};
-        const int32_t* aptr = J.lp->m > 0 ? J.lp->row_ptr : zero_ptr;
-        std::vector<int32_t> arb, trb;
-        group_blocks(*J.h.M, aptr, J.h.n, J.h.m, J.h.nnz, J.h.G, arb, trb);
-        const std::vector<int32_t> ad = block_descriptors(arb, aptr);
-        const std::vector<int32_t> td = block_descriptors(trb, J.h.M->tptr.data());
-        HIP_TRY(ctx, put(W.lay[k].a_rb, ad.data(), ad.size() * 4));
-        HIP_TRY(ctx, put(W.lay[k].t_rb, td.data(), td.size() * 4));
-        std::vector<int32_t> awrb = partition(arb, aptr, J.h.G);
-        std::vector<int32_t> twrb = partition(trb, J.h.M->tptr.data(), J.h.G);
-        HIP_TRY(ctx, put(W.lay[k].a_wrb, awrb.data(), awrb.size() * 4));
-        HIP_TRY(ctx, put(W.lay[k].t_wrb, twrb.data(), twrb.size() * 4));
-      }
+      const int want = next[j];
+      if (want == J.h.G) continue;
+      J.h.G = want;
+      static const int32_t zero_ptr[1] = {0};
+      const int32_t* aptr = J.lp->m > 0 ? J.lp->row_ptr : zero_ptr;
+      std::vector<int32_t> arb, trb;
+      group_blocks(*J.h.M, aptr, J.h.n, J.h.m, J.h.nnz, J.h.G, arb, trb);
+      const std::vector<int32_t> ad = block_descriptors(arb, aptr);
+      const std::vector<int32_t> td = block_descriptors(trb, J.h.M->tptr.data());
+      HIP_TRY(ctx, put(W.lay[k].a_rb, ad.data(), ad.size() * 4));
+      HIP_TRY(ctx, put(W.lay[k].t_rb, td.data(), td.size() * 4));
+      std::vector<int32_t> awrb = partition(arb, aptr, J.h.G);
+      std::vector<int32_t> twrb = partition(trb, J.h.M->tptr.data(), J.h.G);
+      HIP_TRY(ctx, put(W.lay[k].a_wrb, awrb.data(), awrb.size() * 4));
+      HIP_TRY(ctx, put(W.lay[k].t_wrb, twrb.data(), twrb.size() * 4));
       W.views[k].G = J.h.G;
-      for (int g = 0; g < J.h.G; ++g) W.wg_lp.push_back(wg_entry(k, g));
-      base += J.h.G;
     }
-    W.grid = base;
-    place_by_xcd(W.wg_lp, ctx->xcds);
-    HIP_TRY(ctx, hipMemcpy(W.d_views, W.views.data(), W.views.size() * sizeof(LpView),
-                           hipMemcpyHostToDevice));
-    HIP_TRY(ctx, hipMemcpy(W.d_wg_lp, W.wg_lp.data(), W.wg_lp.size() * sizeof(int32_t),
-                           hipMemcpyHostToDevice));
     return PRIMA_OK;
   };
+  auto target_us = [&]() {
+    if (!timed_chunks) return 50000.0;
+    // launches of roughly 50 ms, shorter under a short time limit
+    return std::min(50000.0, std::max(1000.0, p->time_limit_ms * 1000.0 / 8.0)) / model_scale;
+  };
+  for (size_t k = 0; k < nlp; ++k) was_running[k] = running_lp(k);
+  {
+    const int rc = plan(target_us());
+    if (rc) return rc;
+  }
   for (;;) {
-    P.iter_budget = budget;
     auto t_l = clk::now();
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    HIP_TRY(ctx, launch_pdhg(prec, W.d_views, W.d_wg_lp, W.grid, P, ctx->stream));
+    HIP_TRY(ctx, launch_pdhg(prec, W.d_views, W.d_off, W.d_ent, W.grid, P, ctx->stream));
     HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     const double launch_ms = ms_since(t_l);
@@ -1056,6 +1181,10 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
       HIP_TRY(ctx, hipEventElapsedTime(&ev_ms, ctx->ev0, ctx->ev1));
       ctx->stats.kernel_ms += ev_ms;
       ctx->stats.launches += 1;
+    }
+    {
+      const int rc = check_abort();
+      if (rc) return rc;
     }
     {
       int64_t wave_iters = 0, newly = 0;
@@ -1148,14 +1277,17 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
     else if (p->time_limit_ms >= 0 && ms_since(t_solve) >= (double)p->time_limit_ms)
       final_status = PRIMA_TIME_LIMIT;
     if (final_status >= 0) {
-      // finalise the running LPs, then any still waiting for workgroups
-      // (those report their starting point)
+      // finalise the running LPs (each reports its current point)
       KParams F = P;
       F.mode = 1;
       F.final_status = final_status;
+      {
+        const int rc = plan(target_us());
+        if (rc) return rc;
+      }
       for (;;) {
         HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-        HIP_TRY(ctx, launch_pdhg(prec, W.d_views, W.d_wg_lp, W.grid, F, ctx->stream));
+        HIP_TRY(ctx, launch_pdhg(prec, W.d_views, W.d_off, W.d_ent, W.grid, F, ctx->stream));
         HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
         HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
         {
@@ -1164,29 +1296,25 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
           ctx->stats.kernel_ms += ev_ms;
           ctx->stats.launches += 1;
         }
+        {
+          const int rc = check_abort();
+          if (rc) return rc;
+        }
         HIP_TRY(ctx, hipMemcpy(states.data(), W.arena + st_base, nlp * sizeof(SolveState),
                                hipMemcpyDeviceToHost));
-        bool waiting = false;
-        for (size_t k = 0; k < nlp; ++k)
-          waiting |= states[k].status < 0 && !active[k] && !dormant[k];
-        if (!waiting) break;
-        const int rc = regroup();
-        if (rc) return rc;
+        break;
       }
       break;
     }
-    {
-      const int rc = regroup();
-      if (rc) return rc;
+    if (timed_chunks && planned_us > 0.0) {
+      const double r = launch_ms * 1000.0 / planned_us;
+      model_scale = std::min(100.0, std::max(0.01, 0.5 * model_scale + 0.5 * r));
     }
-    if (timed_chunks) {
-      // launches of roughly 50 ms, shorter under a short time limit
-      const double target = std::min(50.0, std::max(1.0, p->time_limit_ms / 8.0));
-      const double per_it = launch_ms / std::max(budget, 1);
-      int nb = (int)std::min(1e7, std::max((double)p->evaluation_frequency, target / per_it));
-      budget = std::max(nb, p->evaluation_frequency);
-    } else if (p->chunk_iterations == 0) {
-      budget = model_budget();
+    {
+      int rc = regroup();
+      if (rc) return rc;
+      rc = plan(target_us());
+      if (rc) return rc;
     }
   }
   const double solve_ms = ms_since(t_solve);
@@ -1244,64 +1372,25 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
   return PRIMA_OK;
 }
 
-// Solve a set of prepared jobs in waves that fit the device's co-resident
-// workgroup capacity.
+// Solve a set of prepared jobs. Every job whose arrays fit in a quarter of
+// free device memory runs in one wave, all admitted at once (LPs on one
+// workgroup share workgroups); a batch too large for that runs in waves of
+// consecutive jobs that do fit, and then a warm pass cannot race a cold twin
+// in another wave: it restarts cold in place instead, and the twins are left
+// out.
 int run_jobs(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const prima_params* p,
              bool certificates = true) {
   const int occ = ctx->occ[prec == 32 ? 0 : 1];
   const int capacity = std::max(1, ctx->cu_count * occ);
-  choose_groups(p, jobs, capacity);
-  // Without a time limit (whose clock starts at admission) one wave holds
-  // every LP whose arrays fit in a quarter of free device memory: LPs beyond
-  // the workgroup capacity wait and take over the workgroups of LPs that
-  // finish, so the batch has one tail instead of one per wave.
-  if (p->time_limit_ms < 0) {
-    std::vector<int> all;
-    Carver cv;
-    std::vector<LpLayout> lay;
-    std::vector<int> owner(jobs.size(), -1);
-    for (size_t j = 0; j < jobs.size(); ++j) {
-      if (jobs[j].h.n == 0 && jobs[j].h.m == 0) continue;
-      for (size_t e = 0; e < all.size(); ++e)
-        if (jobs[all[e]].h.M == jobs[j].h.M && owner[e] < 0) {
-          owner[all.size()] = (int)e;
-          break;
-        }
-      lay.push_back(carve(cv, jobs[j].h, prec, jobs[j].ylo != nullptr,
-                          owner[all.size()] >= 0 ? &lay[owner[all.size()]] : nullptr));
-      all.push_back((int)j);
-    }
-    size_t free_b = 0, total_b = 0;
-    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
-    bool fits = !all.empty() && cv.off <= free_b / 4;
-    for (int j : all) fits &= jobs[j].h.G <= capacity;
-    if (fits) {
-      for (size_t j = 0; j < jobs.size(); ++j)
-        if (jobs[j].h.n == 0 && jobs[j].h.m == 0) {
-          prima_result& r = *jobs[j].res;
-          std::memset(&r, 0, sizeof(r));
-          r.status = PRIMA_OPTIMAL;
-          r.objective = r.dual_objective = jobs[j].lp->offset;
-          r.final_step_size = NAN;
-          r.final_primal_weight = NAN;
-        }
-      return run_wave(ctx, prec, jobs, all, p, certificates, capacity);
-    }
-  }
-  // Waves cannot race a warm pass against a twin in another wave: a warm
-  // pass restarts cold in place instead, and the twins are left out.
-  for (auto& J : jobs)
-    if (J.shadow >= 0) {
-      J.cold_budget = jobs[(size_t)J.shadow].max_iter;
-      J.shadow = -1;
-    }
-  std::vector<int> wave;
-  int used = 0;
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
+  const size_t budget = free_b / 4;
+  // nothing to iterate on: the first evaluation of Pdhg.solve finds every
+  // residual zero and both objectives at the offset (Pdhg.scala:209-212)
+  std::vector<int> work;
   for (size_t j = 0; j < jobs.size(); ++j) {
-    if (jobs[j].shadow_of >= 0) continue;
     if (jobs[j].h.n == 0 && jobs[j].h.m == 0) {
-      // nothing to iterate on: the first evaluation of Pdhg.solve finds every
-      // residual zero and both objectives at the offset (Pdhg.scala:209-212)
+      if (jobs[j].shadow_of >= 0) continue;
       prima_result& r = *jobs[j].res;
       std::memset(&r, 0, sizeof(r));
       r.status = PRIMA_OPTIMAL;
@@ -1310,57 +1399,178 @@ int run_jobs(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const prima_
       r.final_primal_weight = NAN;
       continue;
     }
-    if (used + jobs[j].h.G > capacity && !wave.empty()) {
-      int rc = run_wave(ctx, prec, jobs, wave, p, certificates, capacity);
-      if (rc) return rc;
-      wave.clear();
-      used = 0;
-    }
-    wave.push_back((int)j);
-    used += jobs[j].h.G;
+    work.push_back((int)j);
   }
-  if (!wave.empty()) return run_wave(ctx, prec, jobs, wave, p, certificates, capacity);
+  // arena bytes of each job as the wave carves it (a matrix shared with an
+  // earlier job of the wave is counted once)
+  auto arena_of = [&](const std::vector<int>& w) {
+    Carver cv;
+    std::vector<LpLayout> lay;
+    std::vector<int> owner(w.size(), -1);
+    for (size_t a = 0; a < w.size(); ++a) {
+      for (size_t e = 0; e < a; ++e)
+        if (jobs[w[e]].h.M == jobs[w[a]].h.M && owner[e] < 0) {
+          owner[a] = (int)e;
+          break;
+        }
+      lay.push_back(carve(cv, jobs[w[a]].h, prec, jobs[w[a]].ylo != nullptr,
+                          owner[a] >= 0 ? &lay[owner[a]] : nullptr));
+    }
+    return cv.off + w.size() * (sizeof(SolveState) + 8);
+  };
+  auto groups_for = [&](const std::vector<int>& w) {
+    std::vector<SolveJob*> ptrs;
+    for (int j : w) ptrs.push_back(&jobs[j]);
+    // G_cap depends only on the LP, so twins and their warm passes agree
+    choose_groups(p, ptrs, capacity, prec);
+  };
+  if (work.empty()) return PRIMA_OK;
+  groups_for(work);
+  if (arena_of(work) <= budget) return run_wave(ctx, prec, jobs, work, p, certificates, capacity);
+  for (auto& J : jobs)
+    if (J.shadow >= 0) {
+      J.cold_budget = jobs[(size_t)J.shadow].max_iter;
+      J.shadow = -1;
+    }
+  std::vector<int> wave;
+  for (int j : work) {
+    if (jobs[j].shadow_of >= 0) continue;
+    wave.push_back(j);
+    if (wave.size() > 1 && arena_of(wave) > budget) {
+      wave.pop_back();
+      groups_for(wave);
+      const int rc = run_wave(ctx, prec, jobs, wave, p, certificates, capacity);
+      if (rc) return rc;
+      wave.assign(1, j);
+    }
+  }
+  if (wave.empty()) return PRIMA_OK;
+  groups_for(wave);
+  return run_wave(ctx, prec, jobs, wave, p, certificates, capacity);
+}
+
+int prepare_job(prima_ctx* ctx, SolveJob& J, const prima_lp* lp, const prima_params* p,
+                const prima_warm* warm);
+
+// Runs body(i) for i in [0, count) on up to 16 host threads (the host
+// preparation of a batch is independent per LP).
+template <typename F>
+void parallel_for(size_t count, F body) {
+  const size_t hw = std::max(1u, std::thread::hardware_concurrency());
+  const size_t nt = std::min<size_t>(std::min<size_t>(hw, 16), count);
+  if (nt <= 1) {
+    for (size_t i = 0; i < count; ++i) body(i);
+    return;
+  }
+  std::atomic<size_t> next{0};
+  std::vector<std::thread> pool;
+  for (size_t t = 0; t < nt; ++t)
+    pool.emplace_back([&]() {
+      for (size_t i = next.fetch_add(1); i < count; i = next.fetch_add(1)) body(i);
+    });
+  for (auto& th : pool) th.join();
+}
+
+// Pdhg.scala:272-293: warm vectors must be finite.
+std::string validate_warm(const prima_lp* lp, const prima_warm* warm) {
+  if (!warm) return "";
+  if (warm->primal)
+    for (int32_t i = 0; i < lp->n; ++i)
+      if (!std::isfinite(warm->primal[i])) return "warm start primal vector is not finite";
+  if (warm->dual)
+    for (int32_t r = 0; r < lp->m; ++r)
+      if (!std::isfinite(warm->dual[r])) return "warm start dual vector is not finite";
+  return "";
+}
+
+// Host preparation of a set of jobs: validation, matrix equilibration and
+// transpose (once per distinct matrix, in parallel, through the context's
+// content-addressed cache) and the scaled vectors. jobs[j] gets lps[j] and
+// warms[j] (which may be null). The cache keeps every matrix of the call, so
+// the two passes of a mixed-precision batch and branch-and-bound nodes
+// prepare each matrix once.
+int prepare_jobs(prima_ctx* ctx, std::vector<SolveJob>& jobs,
+                 const std::vector<const prima_lp*>& lps, const prima_params* p,
+                 const std::vector<const prima_warm*>& warms) {
+  const size_t N = jobs.size();
+  std::vector<std::string> err(N);
+  std::vector<uint64_t> keys(N, 0);
+  std::vector<clk::time_point> t0(N);
+  parallel_for(N, [&](size_t j) {
+    t0[j] = clk::now();
+    err[j] = validate_lp(lps[j]);
+    if (err[j].empty()) err[j] = validate_warm(lps[j], warms[j]);
+    if (err[j].empty()) keys[j] = matrix_key(lps[j], p);
+  });
+  for (size_t j = 0; j < N; ++j)
+    if (!err[j].empty()) return fail(ctx, PRIMA_EINVAL, err[j]);
+  // distinct matrices not yet prepared
+  std::vector<std::shared_ptr<const MatrixPrep>> prep(N);
+  std::vector<size_t> todo;
+  for (size_t j = 0; j < N; ++j) {
+    for (const auto& c : ctx->prep_cache)
+      if (c->key == keys[j]) prep[j] = c;
+    if (prep[j]) continue;
+    bool dup = false;
+    for (size_t t : todo) dup |= keys[t] == keys[j];
+    if (!dup) todo.push_back(j);
+  }
+  std::vector<std::shared_ptr<MatrixPrep>> fresh(todo.size());
+  parallel_for(todo.size(), [&](size_t t) {
+    fresh[t] = std::const_pointer_cast<MatrixPrep>(prepare_matrix(lps[todo[t]], p));
+    fresh[t]->key = keys[todo[t]];
+  });
+  for (auto& M : fresh) ctx->prep_cache.push_back(M);
+  for (size_t j = 0; j < N; ++j)
+    if (!prep[j])
+      for (const auto& M : fresh)
+        if (M->key == keys[j]) prep[j] = M;
+  // keep what this call uses, plus the most recent few others
+  {
+    std::vector<std::shared_ptr<const MatrixPrep>> keep;
+    const size_t others = 4;
+    size_t kept_others = 0;
+    for (size_t i = ctx->prep_cache.size(); i-- > 0;) {
+      const auto& c = ctx->prep_cache[i];
+      bool used = false;
+      for (size_t j = 0; j < N && !used; ++j) used = prep[j] == c;
+      bool dup = false;
+      for (const auto& k : keep) dup |= k == c;
+      if (dup) continue;
+      if (used || kept_others++ < others) keep.push_back(c);
+    }
+    std::reverse(keep.begin(), keep.end());
+    ctx->prep_cache.swap(keep);
+  }
+  parallel_for(N, [&](size_t j) {
+    SolveJob& J = jobs[j];
+    const prima_lp* lp = lps[j];
+    J.lp = lp;
+    J.warm = warms[j];
+    HostLp& h = J.h;
+    h.src = lp;
+    h.n = lp->n;
+    h.m = lp->m;
+    h.n_eq = lp->n_eq;
+    h.nnz = lp->nnz;
+    h.M = prep[j];
+    scale_vectors(lp, *h.M, h);
+    h.rhs_norm = norm2(lp->q, lp->m);
+    h.obj_norm = norm2(lp->c, lp->n);
+    J.prep_ms = ms_since(t0[j]);
+  });
   return PRIMA_OK;
 }
 
 int prepare_job(prima_ctx* ctx, SolveJob& J, const prima_lp* lp, const prima_params* p,
                 const prima_warm* warm) {
-  std::string e = validate_lp(lp);
-  if (!e.empty()) return fail(ctx, PRIMA_EINVAL, e);
-  if (warm) {
-    // Pdhg.scala:272-293: warm vectors must be finite
-    if (warm->primal)
-      for (int32_t i = 0; i < lp->n; ++i)
-        if (!std::isfinite(warm->primal[i]))
-          return fail(ctx, PRIMA_EINVAL, "warm start primal vector is not finite");
-    if (warm->dual)
-      for (int32_t r = 0; r < lp->m; ++r)
-        if (!std::isfinite(warm->dual[r]))
-          return fail(ctx, PRIMA_EINVAL, "warm start dual vector is not finite");
-  }
-  auto t0 = clk::now();
-  J.lp = lp;
-  J.warm = warm;
-  HostLp& h = J.h;
-  h.src = lp;
-  h.n = lp->n;
-  h.m = lp->m;
-  h.n_eq = lp->n_eq;
-  h.nnz = lp->nnz;
-  const uint64_t key = matrix_key(lp, p);
-  for (const auto& c : ctx->prep_cache)
-    if (c->key == key) h.M = c;
-  if (!h.M) {
-    auto M = prepare_matrix(lp, p);
-    const_cast<MatrixPrep&>(*M).key = key;
-    ctx->prep_cache.push_back(M);
-    if (ctx->prep_cache.size() > 4) ctx->prep_cache.erase(ctx->prep_cache.begin());
-    h.M = M;
-  }
-  scale_vectors(lp, *h.M, h);
-  h.rhs_norm = norm2(lp->q, lp->m);
-  h.obj_norm = norm2(lp->c, lp->n);
-  J.prep_ms = ms_since(t0);
+  std::vector<SolveJob> one(1);
+  const int rc = prepare_jobs(ctx, one, {lp}, p, {warm});
+  if (rc) return rc;
+  J.lp = one[0].lp;
+  J.warm = one[0].warm;
+  J.h = std::move(one[0].h);
+  J.prep_ms = one[0].prep_ms;
   return PRIMA_OK;
 }
 
@@ -1405,9 +1615,17 @@ int mixed_batch(prima_ctx* ctx, int32_t count, const prima_lp* lps, const prima_
   std::vector<prima_result> red(count);
   {
     std::vector<SolveJob> jobs(count);
-    for (int32_t k = 0; k < count; ++k) {
-      int rc = prepare_job(ctx, jobs[k], &lps[k], &rp, warms ? &warms[k] : nullptr);
+    {
+      std::vector<const prima_lp*> lp_of(count);
+      std::vector<const prima_warm*> warm_of(count);
+      for (int32_t k = 0; k < count; ++k) {
+        lp_of[k] = &lps[k];
+        warm_of[k] = warms ? &warms[k] : nullptr;
+      }
+      int rc = prepare_jobs(ctx, jobs, lp_of, &rp, warm_of);
       if (rc) return rc;
+    }
+    for (int32_t k = 0; k < count; ++k) {
       rx[k].assign((size_t)lps[k].n, 0.0);
       ry[k].assign((size_t)lps[k].m, 0.0);
       rrc[k].assign((size_t)lps[k].n, 0.0);
@@ -1470,11 +1688,19 @@ int mixed_batch(prima_ctx* ctx, int32_t count, const prima_lp* lps, const prima_
   const bool twins = stall && fp.time_limit_ms < 0;
   std::vector<SolveJob> jobs(twins ? 2 * N : N);
   const prima_warm none{nullptr, nullptr, NAN, NAN};
+  {
+    std::vector<const prima_lp*> lp_of(jobs.size());
+    std::vector<const prima_warm*> warm_of(jobs.size());
+    for (size_t j = 0; j < jobs.size(); ++j) {
+      lp_of[j] = &lps[which[j % N]];
+      warm_of[j] = j >= N ? &none : &rw[which[j % N]];
+    }
+    int rc = prepare_jobs(ctx, jobs, lp_of, &fp, warm_of);
+    if (rc) return rc;
+  }
   for (size_t j = 0; j < jobs.size(); ++j) {
     const int k = which[j % N];
     const bool twin = j >= N;
-    int rc = prepare_job(ctx, jobs[j], &lps[k], &fp, twin ? &none : &rw[k]);
-    if (rc) return rc;
     jobs[j].res = &results[k];
     jobs[j].x_out = xo ? xo[k] : nullptr;
     jobs[j].y_out = yo ? yo[k] : nullptr;
@@ -1606,9 +1832,17 @@ int refine_batch(prima_ctx* ctx, int32_t count, const prima_lp* lps, const prima
   std::vector<prima_result> red(count);
   {
     std::vector<SolveJob> jobs(count);
-    for (int32_t k = 0; k < count; ++k) {
-      int rc = prepare_job(ctx, jobs[k], &lps[k], &rp, warms ? &warms[k] : nullptr);
+    {
+      std::vector<const prima_lp*> lp_of(count);
+      std::vector<const prima_warm*> warm_of(count);
+      for (int32_t k = 0; k < count; ++k) {
+        lp_of[k] = &lps[k];
+        warm_of[k] = warms ? &warms[k] : nullptr;
+      }
+      int rc = prepare_jobs(ctx, jobs, lp_of, &rp, warm_of);
       if (rc) return rc;
+    }
+    for (int32_t k = 0; k < count; ++k) {
       S[k].x.assign((size_t)lps[k].n, 0.0);
       S[k].y.assign((size_t)lps[k].m, 0.0);
       S[k].crc.assign((size_t)lps[k].n, 0.0);
@@ -2219,6 +2453,13 @@ int prima_cancel(prima_ctx* ctx) {
   return PRIMA_OK;
 }
 
+int prima_test_stall_barriers(prima_ctx* ctx, int32_t enable) {
+  int rc = check_open(ctx);
+  if (rc) return rc;
+  ctx->test_stall = enable ? 1 : 0;
+  return PRIMA_OK;
+}
+
 int prima_vec_alloc(prima_ctx* ctx, int64_t n, prima_vec** out) {
   int rc = check_open(ctx);
   if (rc) return rc;
@@ -2658,9 +2899,17 @@ int prima_solve_batch(prima_ctx* ctx, int32_t count, const prima_lp* lps,
     return PRIMA_OK;
   }
   std::vector<SolveJob> jobs(count);
-  for (int32_t k = 0; k < count; ++k) {
-    rc = prepare_job(ctx, jobs[k], &lps[k], params, warms ? &warms[k] : nullptr);
+  {
+    std::vector<const prima_lp*> lp_of(count);
+    std::vector<const prima_warm*> warm_of(count);
+    for (int32_t k = 0; k < count; ++k) {
+      lp_of[k] = &lps[k];
+      warm_of[k] = warms ? &warms[k] : nullptr;
+    }
+    rc = prepare_jobs(ctx, jobs, lp_of, params, warm_of);
     if (rc) return rc;
+  }
+  for (int32_t k = 0; k < count; ++k) {
     jobs[k].res = &results[k];
     jobs[k].x_out = x_outs ? x_outs[k] : nullptr;
     jobs[k].y_out = y_outs ? y_outs[k] : nullptr;

@@ -187,8 +187,8 @@ __global__ __launch_bounds__(NT) void k_spmv_blocks(const int32_t* __restrict__ 
                                                     const T* __restrict__ x,
                                                     const int32_t* __restrict__ rbd, int nblocks,
                                                     int bpw, T* __restrict__ out) {
-  __shared__ double tile[TILE];
-  __shared__ double rsum[ROWCAP];
+  __shared__ T tile[TILE];
+  __shared__ T rsum[ROWCAP];
   __shared__ int rptr[ROWCAP + 1];
   const int b0 = blockIdx.x * bpw;
   const int b1 = b0 + bpw < nblocks ? b0 + bpw : nblocks;

@@ -103,6 +103,30 @@ __device__ inline void group_sum(const double* part, int stride, int G, double (
   __syncthreads();
 }
 
+// Poll `flag` (one thread) until it reads `expect`. Every 1024 polls the
+// group's abort word bar[2] is read: if another workgroup set it, give up.
+// After `limit` polls set the abort word (so the rest of the group gives up
+// too) and give up. Returns false when it gave up.
+__device__ inline bool bounded_wait(uint32_t* flag, uint32_t expect, uint32_t* bar,
+                                    uint64_t limit) {
+  const uint32_t zero = (uint32_t)(limit >> 63);  // 0, opaque to the compiler
+  uint64_t spins = 0;
+  while (__hip_atomic_fetch_add(flag, zero, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) !=
+         expect) {
+    __builtin_amdgcn_s_sleep(2);
+    if ((++spins & 1023u) == 0) {
+      if (__hip_atomic_fetch_add(&bar[2], zero, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) !=
+          0u)
+        return false;
+      if (spins > limit) {
+        __hip_atomic_exchange(&bar[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return false;
+      }
+    }
+  }
+  return true;
+}
+
 // Group barrier over the G workgroups of one LP. Words: bar[0] arrivals,
 // bar[1] completed barriers (read at launch start into *epoch), bar[2]
 // abort; bar[32 + 32 j] the release flag of rank j, each on its own 128-byte
@@ -146,23 +170,7 @@ __device__ inline bool group_barrier(uint32_t* bar, int G, int rank, uint32_t* e
         if (j != rank)
           __hip_atomic_exchange(&bar[32 + 32 * j], e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     } else if (threadIdx.x == 0) {
-      uint64_t spins = 0;
-      uint32_t* flag = &bar[32 + 32 * rank];
-      while (__hip_atomic_fetch_add(flag, zero, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != e) {
-        __builtin_amdgcn_s_sleep(2);
-        if ((++spins & 1023u) == 0) {
-          if (__hip_atomic_fetch_add(&bar[2], zero, __ATOMIC_RELAXED,
-                                     __HIP_MEMORY_SCOPE_AGENT) != 0u) {
-            ok = 0;
-            break;
-          }
-          if (spins > (1ull << 26)) {
-            __hip_atomic_exchange(&bar[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            ok = 0;
-            break;
-          }
-        }
-      }
+      ok = bounded_wait(&bar[32 + 32 * rank], e, bar, 1ull << 26) ? 1 : 0;
     }
     if (threadIdx.x == 0) {
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
@@ -506,9 +514,13 @@ struct Ranges {
   int rank, G;
 };
 
+// Workgroup scratch in LDS, typed on the iteration precision: the streamed
+// products and row sums of a block (TILE and ROWCAP entries of T) and its
+// row pointers, plus the fixed-order reduction slots.
+template <typename T>
 struct Smem {
-  double tile[TILE];
-  double rsum[ROWCAP];
+  T tile[TILE];
+  T rsum[ROWCAP];
   int rptr[ROWCAP + 1];
   double red[NWAVE * kEvalParts + kEvalParts];
   double E[kEvalParts];  // group totals of the last evaluation
@@ -520,7 +532,7 @@ struct Smem {
 // (Pdhg.scala:399). Returns false if a group barrier aborted.
 template <typename T>
 __device__ __noinline__ bool eval_pass(const LpView& v, const Ranges& R, bool fresh_kx, bool avg,
-                                       double wsum, Smem& sm) {
+                                       double wsum, Smem<T>& sm) {
   const Lp<T> L(v);
   const T inv = avg ? (T)(1.0 / wsum) : T(0);
   double e[E_COUNT];
@@ -529,8 +541,7 @@ __device__ __noinline__ bool eval_pass(const LpView& v, const Ranges& R, bool fr
   if (fresh_kx) {
     EpiFreshKx<T> epi{L, avg, inv, e};
     spmv_blocks(v.a_ptr, v.a_col, L.aval(), (const T*)L.x(), v.a_rb, R.ab0, R.ab1,
-                reinterpret_cast<T*>(sm.tile),
-                reinterpret_cast<T*>(sm.rsum), sm.rptr, epi);
+                sm.tile, sm.rsum, sm.rptr, epi);
   } else {
     for (int r = R.r0 + threadIdx.x; r < R.r1; r += NT)
       m_eval(v, r, L.kx()[r], m_ops(L, r, avg), avg, inv, e);
@@ -612,18 +623,17 @@ __device__ __noinline__ void restart_arrays(const LpView& v, const Ranges& R, bo
 // optionally followed by the next primal step; returns this thread's |dx|^2.
 template <typename T>
 __device__ inline T commit_columns(const LpView& v, const Ranges& R, T w, bool primal, T tau,
-                                   Smem& sm) {
+                                   Smem<T>& sm) {
   const Lp<T> L(v);
   EpiCommitPrimal<T> epi{L, w, tau, primal, T(0)};
   spmv_blocks(v.t_ptr, v.t_col, L.tval(), (const T*)L.yn(), v.t_rb, R.tb0, R.tb1,
-              reinterpret_cast<T*>(sm.tile),
-                reinterpret_cast<T*>(sm.rsum), sm.rptr, epi);
+              sm.tile, sm.rsum, sm.rptr, epi);
   return epi.dx2;
 }
 
 template <typename T>
 __device__ __noinline__ void commit_columns_cold(const LpView& v, const Ranges& R, T w,
-                                                 Smem& sm) {
+                                                 Smem<T>& sm) {
   (void)commit_columns<T>(v, R, w, false, T(0), sm);
 }
 
@@ -646,12 +656,11 @@ __device__ __noinline__ void commit_rows_cold(const LpView& v, const Ranges& R, 
 // Initialisation on the device (Pdhg.scala:305-309): kty = A^T y, restart
 // point, zero running sums.
 template <typename T>
-__device__ __noinline__ void init_arrays(const LpView& v, const Ranges& R, Smem& sm) {
+__device__ __noinline__ void init_arrays(const LpView& v, const Ranges& R, Smem<T>& sm) {
   const Lp<T> L(v);
   EpiInitKty<T> epi{L};
   spmv_blocks(v.t_ptr, v.t_col, L.tval(), (const T*)L.y(), v.t_rb, R.tb0, R.tb1,
-              reinterpret_cast<T*>(sm.tile),
-                reinterpret_cast<T*>(sm.rsum), sm.rptr, epi);
+              sm.tile, sm.rsum, sm.rptr, epi);
   for (int r = R.r0 + threadIdx.x; r < R.r1; r += NT) {
     L.yr()[r] = L.y()[r];
     L.sy()[r] = T(0);
@@ -676,9 +685,14 @@ __device__ __noinline__ void finish(const LpView& v, const Ranges& R, SolveState
   store_state(v, R, S);
 }
 
-__device__ __noinline__ void abort_launch(const LpView& v, const Ranges& R, SolveState& S) {
-  S.status = 5;  // NumericalError: a group barrier timed out
-  store_state(v, R, S);
+// A group barrier gave up (a workgroup of the group never arrived, or
+// another gave up first). The LP's state is left as it was at the launch's
+// start; the launch-wide abort word records the LP (index + 1) so that the
+// host reports a device error instead of a solver outcome.
+__device__ __noinline__ void abort_launch(const LpView& v, int lp) {
+  if (threadIdx.x == 0)
+    __hip_atomic_exchange(v.launch_abort, (uint32_t)lp + 1u, __ATOMIC_RELAXED,
+                          __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // The checkpoint decision (Pdhg.scala:396-417, 463-502) from the group
@@ -686,7 +700,7 @@ __device__ __noinline__ void abort_launch(const LpView& v, const Ranges& R, Solv
 // restart the arrays are re-centred.
 template <typename T>
 __device__ __noinline__ int checkpoint_decide(const LpView& v, const Ranges& R, const KParams& P,
-                                              SolveState& S, bool have_avg, Smem& sm,
+                                              SolveState& S, bool have_avg, Smem<T>& sm,
                                               bool* avg_out, Kkt* err_out) {
   const double* E = sm.E;
   const Kkt cur = kkt_current(E, v.offset);
@@ -732,15 +746,14 @@ __device__ __noinline__ int checkpoint_decide(const LpView& v, const Ranges& R, 
   return -1;
 }
 
+// One LP's chunk of a launch, run by one workgroup of its group: `me` is
+// the LP index << 9 | the workgroup's rank in the group.
 template <typename T>
-__global__ __launch_bounds__(NT) void pdhg_persistent(const LpView* __restrict__ views,
-                                                      const int32_t* __restrict__ wg_lp,
-                                                      KParams P) {
-  __shared__ Smem sm;
-  T* tile = reinterpret_cast<T*>(sm.tile);
-
-  const int32_t me = wg_lp[blockIdx.x];  // LP index << 9 | rank in its group
-  const LpView& v = views[me >> 9];
+__device__ inline void run_chunk(const LpView* __restrict__ views, int32_t me, const KParams& P,
+                                 Smem<T>& sm) {
+  T* tile = sm.tile;
+  const int lp = me >> 9;
+  const LpView& v = views[lp];
   const Lp<T> L(v);
   Ranges R;
   R.G = v.G;
@@ -765,24 +778,33 @@ __global__ __launch_bounds__(NT) void pdhg_persistent(const LpView* __restrict__
   if (S.status >= 0) return;
   double* ipart = v.part;  // [2][G][kIterParts]
 
+  if (P.test_stall && G > 1 && R.rank == G - 1) {
+    // test hook: the last rank waits for bar[3] (a word nothing writes) to
+    // read 1, with a short limit, so that its bounded wait gives up, sets
+    // the group's abort word and the rest of the group gives up in turn
+    if (threadIdx.x == 0) (void)bounded_wait(&v.bar[3], 1u, v.bar, 1u << 14);
+    __syncthreads();
+    return abort_launch(v, lp);
+  }
+
   if (P.mode == 1) {
     // finalise: an LP that never ran (still waiting for workgroups) is
     // initialised first and reports its starting point
     const bool fresh = S.status == -2;
     if (fresh) {
       init_arrays<T>(v, R, sm);
-      if (!group_barrier(v.bar, G, R.rank, &sm.epoch)) return abort_launch(v, R, S);
+      if (!group_barrier(v.bar, G, R.rank, &sm.epoch)) return abort_launch(v, lp);
       S.wsum = 0.0;
     }
-    if (!eval_pass<T>(v, R, fresh, false, S.wsum, sm)) return abort_launch(v, R, S);
+    if (!eval_pass<T>(v, R, fresh, false, S.wsum, sm)) return abort_launch(v, lp);
     return finish<T>(v, R, S, P.final_status, kkt_current(sm.E, v.offset), false);
   }
 
   if (S.status == -2) {
     init_arrays<T>(v, R, sm);
-    if (!group_barrier(v.bar, G, R.rank, &sm.epoch)) return abort_launch(v, R, S);
+    if (!group_barrier(v.bar, G, R.rank, &sm.epoch)) return abort_launch(v, lp);
     S.wsum = 0.0;
-    if (!eval_pass<T>(v, R, true, false, 0.0, sm)) return abort_launch(v, R, S);
+    if (!eval_pass<T>(v, R, true, false, 0.0, sm)) return abort_launch(v, lp);
     const Kkt cur = kkt_current(sm.E, v.offset);
     if (kkt_converged(cur, v, P)) return finish<T>(v, R, S, 0, cur, false);
     S.err_at_restart = kkt_weighted(cur, S.weight);
@@ -814,13 +836,13 @@ __global__ __launch_bounds__(NT) void pdhg_persistent(const LpView* __restrict__
       block_sum<1>(a, sm.red);
       if (threadIdx.x == 0) ip[0] = a[0];
     }
-    if (!group_barrier(v.bar, G, R.rank, &sm.epoch)) return abort_launch(v, R, S);
+    if (!group_barrier(v.bar, G, R.rank, &sm.epoch)) return abort_launch(v, lp);
 
     // phase 2: own rows
     {
       EpiDual<T> epi{L, sigma, v.n_eq, T(0), T(0), pending_rows, wpend};
       spmv_blocks(v.a_ptr, v.a_col, L.aval(), (const T*)L.xbar(), v.a_rb, R.ab0, R.ab1, tile,
-                  reinterpret_cast<T*>(sm.rsum), sm.rptr, epi);
+                  sm.rsum, sm.rptr, epi);
       pending_rows = false;
       double a[2] = {(double)epi.dy2, (double)epi.dykdx};
       block_sum<2>(a, sm.red);
@@ -829,7 +851,7 @@ __global__ __launch_bounds__(NT) void pdhg_persistent(const LpView* __restrict__
         ip[2] = a[1];
       }
     }
-    if (!group_barrier(v.bar, G, R.rank, &sm.epoch)) return abort_launch(v, R, S);
+    if (!group_barrier(v.bar, G, R.rank, &sm.epoch)) return abort_launch(v, lp);
 
     // decide (identical in every workgroup)
     double tot[3];
@@ -843,7 +865,7 @@ __global__ __launch_bounds__(NT) void pdhg_persistent(const LpView* __restrict__
       // fixed point (Pdhg.scala:221-231): fresh K x, then the usual test
       S.iteration++;
       S.trials = 0;
-      if (!eval_pass<T>(v, R, true, false, S.wsum, sm)) return abort_launch(v, R, S);
+      if (!eval_pass<T>(v, R, true, false, S.wsum, sm)) return abort_launch(v, lp);
       const Kkt cur = kkt_current(sm.E, v.offset);
       return finish<T>(v, R, S, kkt_converged(cur, v, P) ? 0 : 5, cur, false);
     }
@@ -875,9 +897,9 @@ __global__ __launch_bounds__(NT) void pdhg_persistent(const LpView* __restrict__
       pending_rows = false;
       commit_columns_cold<T>(v, R, wpend, sm);
       pending = false;
-      if (!group_barrier(v.bar, G, R.rank, &sm.epoch)) return abort_launch(v, R, S);
+      if (!group_barrier(v.bar, G, R.rank, &sm.epoch)) return abort_launch(v, lp);
       const bool have_avg = S.wsum > 0.0;
-      if (!eval_pass<T>(v, R, true, have_avg, S.wsum, sm)) return abort_launch(v, R, S);
+      if (!eval_pass<T>(v, R, true, have_avg, S.wsum, sm)) return abort_launch(v, lp);
       S.evals++;
       evaluated_now = true;
       bool avg = false;
@@ -895,15 +917,42 @@ __global__ __launch_bounds__(NT) void pdhg_persistent(const LpView* __restrict__
         commit_columns_cold<T>(v, R, wpend, sm);
         pending = false;
       }
-      __syncthreads();
-      if (!eval_pass<T>(v, R, false, false, S.wsum, sm)) return abort_launch(v, R, S);
+      // The evaluation partials are single-buffered: when the checkpoint
+      // above has just reduced them, every rank must be done reading its
+      // group's slots before any rank overwrites its own again.
+      if (evaluated_now) {
+        if (!group_barrier(v.bar, G, R.rank, &sm.epoch)) return abort_launch(v, lp);
+      } else {
+        __syncthreads();
+      }
+      if (!eval_pass<T>(v, R, false, false, S.wsum, sm)) return abort_launch(v, lp);
       return finish<T>(v, R, S, 3, kkt_current(sm.E, v.offset), false);
     }
 
-    if (evaluated_now && S.iteration - it_start >= P.iter_budget) {
+    if (evaluated_now && S.iteration - it_start >= v.iter_budget) {
       // end of this launch's chunk: state is fully committed
       return store_state(v, R, S);
     }
+  }
+}
+
+// Every workgroup runs the chunks its list names, in order: first the LP
+// whose group it belongs to, if any (so every group's workgroups start on it
+// together), then any number of LPs it runs alone.
+#ifndef PRIMA_WAVES_PER_EU
+#define PRIMA_WAVES_PER_EU 2
+#endif
+template <typename T>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PRIMA_WAVES_PER_EU)))
+void pdhg_persistent(const LpView* __restrict__ views,
+                                                      const int32_t* __restrict__ wg_off,
+                                                      const int32_t* __restrict__ wg_ent,
+                                                      KParams P) {
+  __shared__ Smem<T> sm;
+  const int e1 = wg_off[blockIdx.x + 1];
+  for (int e = wg_off[blockIdx.x]; e < e1; ++e) {
+    run_chunk<T>(views, wg_ent[e], P, sm);
+    __syncthreads();
   }
 }
 
@@ -920,9 +969,9 @@ int occupancy_blocks_per_cu(int precision) {
   return e == hipSuccess ? nb : 0;
 }
 
-hipError_t launch_pdhg(int precision, const LpView* views, const int32_t* wg_lp, int grid,
-                       const KParams& P, hipStream_t stream) {
-  void* args[] = {(void*)&views, (void*)&wg_lp, (void*)&P};
+hipError_t launch_pdhg(int precision, const LpView* views, const int32_t* wg_off,
+                       const int32_t* wg_ent, int grid, const KParams& P, hipStream_t stream) {
+  void* args[] = {(void*)&views, (void*)&wg_off, (void*)&wg_ent, (void*)&P};
   if (precision == 32)
     return hipLaunchCooperativeKernel((const void*)pdhg_persistent<float>, dim3(grid), dim3(NT),
                                       args, 0, stream);

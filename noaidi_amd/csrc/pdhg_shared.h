@@ -30,10 +30,10 @@ struct KParams {
   double eps_abs, eps_rel, infeas_tol;
   double r_suff, r_nec, r_art, theta;
   int32_t eval_freq;     // iteration caps are per LP (LpView::max_iter)
-  int32_t iter_budget;   // stop after a checkpoint once this many iterations ran
   int32_t mode;          // 0: run, 1: finalize with final_status
   int32_t final_status;
   int32_t certificates;  // 1: test Farkas certificates at checkpoints
+  int32_t test_stall;    // test hook: make every group's barrier wait give up
 };
 
 // Device view of one LP (one group of workgroups). Pointers are device
@@ -42,6 +42,7 @@ struct LpView {
   int32_t n, m, n_eq;
   int32_t G;         // workgroups in this group
   int32_t max_iter;  // this LP's iteration cap
+  int32_t iter_budget;  // end this LP's chunk after a checkpoint once this many ran
   const int32_t *a_ptr, *a_col;
   const void* a_val;
   const int32_t *t_ptr, *t_col;
@@ -60,6 +61,7 @@ struct LpView {
   double* part;       // 2 * G * 4 iteration partials, then G * kEvalParts
   uint32_t* bar;      // barrier words: count, generation, abort, then a flag line per rank
   SolveState* st;
+  uint32_t* launch_abort;  // set (LP index + 1) when a group barrier gives up
   double offset, rhs_norm, obj_norm, spec_orig;
 };
 

@@ -8,14 +8,25 @@ namespace prima {
 
 #ifndef PRIMA_NT
 #define PRIMA_NT 512
+#endif
+#ifndef PRIMA_TILE
 #define PRIMA_TILE 8192
+#endif
+#ifndef PRIMA_ROWCAP
 #define PRIMA_ROWCAP 4096
+#endif
+#ifndef PRIMA_DEPTH
+#define PRIMA_DEPTH 16
+#endif
+#ifndef PRIMA_EPI_ROWS
+#define PRIMA_EPI_ROWS 4
 #endif
 constexpr int NT = PRIMA_NT;          // threads per workgroup
 constexpr int NWAVE = NT / 64;
 constexpr int TILE = PRIMA_TILE;      // nnz of a streamed row block (staged in LDS)
 constexpr int ROWCAP = PRIMA_ROWCAP;  // rows of a streamed row block
-constexpr int DEPTH = 16;     // independent product loads in flight per thread
+constexpr int DEPTH = PRIMA_DEPTH;    // independent product loads in flight per thread
+constexpr int EPI_ROWS = PRIMA_EPI_ROWS;  // rows whose epilogue operands are in flight
 
 // ---------------------------------------------------------------------------
 // CSR SpMV over a workgroup's row blocks (CSR-Adaptive style). A block of
@@ -106,13 +117,13 @@ __device__ inline void spmv_blocks(const int32_t* __restrict__ ptr, const int32_
       if (li == 0) rsum[r] = acc;
     }
     __syncthreads();
-    for (int base = threadIdx.x; base < nrows; base += 4 * NT) {
-      typename Epi::Ops o[4];
+    for (int base = threadIdx.x; base < nrows; base += EPI_ROWS * NT) {
+      typename Epi::Ops o[EPI_ROWS];
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
+      for (int j = 0; j < EPI_ROWS; ++j)
         if (base + j * NT < nrows) o[j] = epi.load(r0 + base + j * NT);
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
+      for (int j = 0; j < EPI_ROWS; ++j)
         if (base + j * NT < nrows) epi.apply(r0 + base + j * NT, rsum[base + j * NT], o[j]);
     }
     __syncthreads();

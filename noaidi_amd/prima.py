@@ -157,7 +157,7 @@ EXPORTED = [
     "prima_mixed_solve", "prima_solve_batch", "prima_cancel", "prima_stats_get",
     "prima_stats_reset", "prima_mat_set_schedule", "prima_mat_autotune",
     "prima_default_bnb_params", "prima_milp_solve", "prima_bnb_skips_before_solving",
-    "prima_bnb_safe_bound_of", "prima_progress",
+    "prima_bnb_safe_bound_of", "prima_progress", "prima_test_stall_barriers",
 ]
 
 _lib = None
@@ -216,6 +216,7 @@ def lib():
             "prima_bnb_skips_before_solving": (i32, [d, d, i32]),
             "prima_bnb_safe_bound_of": (d, [i32, d, d]),
             "prima_progress": (i32, [vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(i64)]),
+            "prima_test_stall_barriers": (i32, [vp, i32]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -786,6 +787,13 @@ class HipKernels:
 
     def cancel(self) -> None:
         lib().prima_cancel(self._h)
+
+    def _test_stall_barriers(self, enable: bool) -> None:
+        """Test hook (no reference counterpart): while enabled, solves whose
+        LPs span several workgroups fail with DeviceError, as when a group
+        barrier of the solver kernel gives up."""
+        self._check(lib().prima_test_stall_barriers(self._h, 1 if enable else 0),
+                    "test_stall_barriers")
 
     def stats(self) -> dict:
         st = _Stats()
