@@ -1,41 +1,52 @@
 """Benchmark: LPs/sec to 1e-8 relative KKT on one MI355X.
 
-Workload: the 200-bus, 24-snapshot PyPSA-shaped dispatch LP (34,800
-variables, 14,400 rows, 64,750 nonzeros), as a batch of demand scenarios
-(per-bus demand x U[0.8, 1.2], scenario seeds 1000..1000+B-1). One step
-solves the whole batch with mixed precision (float32 device pass, then a
-float64 device pass warm-started from it) inside shared persistent launches.
-Plain FP64 PDHG solving the first --unbatched scenarios one at a time is timed
-beside it: the metric's denominator. LPs that finish early hand their
-workgroups to the ones still running (elastic batching).
+Headline workload (the default, --workload scenarios): 256 demand scenarios
+of the 200-bus, 24-snapshot PyPSA-shaped dispatch LP (the config-1 LP: 34,800
+variables, 14,400 rows, 64,750 nonzeros; per-bus demand x U[0.8, 1.2] with
+seed 1000 + k). One step solves all 256 in one batched call with mixed
+precision: a float32 device pass, then a float64 device pass warm-started
+from it (the reference's MixedPrecision contract), every LP in the same
+persistent launches with its own step size, restarts and termination. Every
+LP must come back Optimal and within 1e-8 (|a-b|/max(1,|b|)) of its HiGHS
+objective (tests/golden/highs_dispatch.json), or the run fails.
 
-Every LP must come back Optimal with an objective within 1e-8 (relative,
-|a-b|/max(1,|b|)) of the HiGHS golden in tests/golden/highs_dispatch.json;
-otherwise the run fails.
+--workload sweep runs the same measurement on the uneven scenario sweep
+(prima.config3, k = 0..--batch-1: buses log-uniform in [50, 2000], snapshots
+in {24, 72, 168}; 88.1M variables over 256 LPs). To 1e-8 its largest LPs need
+far more than a million float64 iterations each, which is longer than the
+benchmark's time limit.
 
-  python bench.py [--steps K] [--warmup W] [--batch B] [--no-cpu]
+Timing follows the driver's fixed command (--steps K --warmup W): warm-up is
+capped at one small solve per context (the persistent kernel has no JIT to
+warm), and steps are timed until the wall budget (--budget seconds) would be
+exceeded, at least one; `steps` reports the steps actually timed, and a
+repeated step must reproduce the first bit for bit. Then, in the same run:
+the whole batch in plain FP64 batched; plain FP64 one LP at a time (the
+metric's denominator) on a sample stratified by the FP64 iteration counts
+(by size for the sweep); and the oracle (the C restatement of Prima's FP64
+CPU path) on the host cores. One JSON line, printed and flushed at the end.
 
---workload kernel runs the solver kernel alone for profiling: --batch LPs of
-the dispatch form for --iterations accepted iterations each (tolerance 1e-12,
-so nothing stops early) in float64 and then float32, after a plain copy of
-2^26 doubles through the seam's copy op (512 MiB read, 512 MiB written) that
-calibrates hardware byte counters. It prints the kernel's algorithmic bytes
-per launch and event-timed launch durations.
+  python bench.py [--steps K] [--warmup W] [--budget S] [--no-cpu]
+                  [--workload scenarios|sweep|uc-frontier|kernel|lopf|spmv]
 
---workload uc-frontier measures the branch-and-bound frontier instead: the
-100-bus, 24-snapshot unit-commitment relaxation (37,200 vars, 49,200 rows,
-154,800 nnz) and --batch nodes of its depth-9 frontier (nine commitment
-binaries fixed per node, same K), warm-started from the root's solution and
-solved in one batched call, mixed precision, checked against HiGHS.
+--workload uc-frontier: the 100-bus UC relaxation's depth-9 branch-and-bound
+frontier (--batch nodes, shared K, warm-started from the root). --workload
+kernel: the solver kernel alone for profiling (dispatch LPs for --iterations
+iterations each), after a 2^26-double copy that calibrates hardware byte
+counters. --workload lopf: PyPSA's scigrid-de. --workload spmv: the SpMV
+schedules on the 2M-column random LP.
 """
 from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import platform
 import sys
+import threading
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -44,30 +55,15 @@ sys.path.insert(0, ROOT)
 
 from noaidi_amd import prima  # noqa: E402
 
+METRIC = "LPs/sec at 1e-8 rel. KKT: mixed-prec batched vs plain FP64 PDHG, 1 MI355X"
 PEAK_HBM_GBS = 8000.0
-# HBM bytes per algorithmic byte of pdhg_persistent, measured with rocprofv3
-# FETCH_SIZE and WRITE_SIZE (calibrated on a 512 MiB device copy): over the
-# mixed step of this benchmark's own command (profiles/
-# bench_dispatch200x24_batch256_pmc.md), and for the float64 / float32
-# kernels on 256 dispatch LPs at 2,048 iterations each (profiles/
-# pdhg_persistent_dispatch200x24_batch256_kernel.md)
-TRAFFIC_PER_ALGORITHMIC_BYTE = {"mixed": 0.84, 64: 0.81, 32: 0.75}
-GOLDEN = os.path.join(ROOT, "tests", "golden", "highs_dispatch.json")
-
-
-def scenario(k: int) -> prima.LpProblem:
-    return prima.dispatch(seed=1, perturb=0.2, scenario_seed=1000 + k)
-
-
-def golden_objectives() -> dict:
-    with open(GOLDEN) as f:
-        data = json.load(f)
-    out = {}
-    for d in data:
-        a = d["args"]
-        if a.get("seed") == 1 and a.get("perturb") == 0.2 and "buses" not in a:
-            out[a["scenario_seed"] - 1000] = d["objective"]
-    return out
+GOLDEN_DIR = os.path.join(ROOT, "tests", "golden")
+# HBM bytes per algorithmic byte of pdhg_persistent, from rocprofv3
+# FETCH_SIZE and WRITE_SIZE scaled by the counters' reading of a 512 MiB
+# device copy (profiles/pdhg_persistent_config3_sweep_pmc.md)
+TRAFFIC_PER_ALGORITHMIC_BYTE = {32: None, 64: None}
+# host cores of the GPU box's share for one GPU
+CPU_CORES = 16
 
 
 def log(*a):
@@ -78,114 +74,27 @@ def rel_gap(a, b):
     return abs(a - b) / max(1.0, abs(b))
 
 
-def check(sols, idx, gold, what):
-    worst = 0.0
-    for k, s in zip(idx, sols):
-        if s.status != "Optimal":
-            raise SystemExit(f"{what}: scenario {k} ended {s.status}")
-        if k in gold:
-            g = rel_gap(s.objective_value, gold[k])
-            worst = max(worst, g)
-            if g > 1e-8:
-                raise SystemExit(f"{what}: scenario {k} objective {s.objective_value!r} vs "
-                                 f"HiGHS {gold[k]!r} (gap {g:.2e})")
-    return worst
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
 
 
-# Iterations the oracle needs to bring scenario 0 to 1e-8 (deterministic:
-# same restatement, same libm). The baseline times a bounded prefix of that
-# run and scales it to one full LP.
-ORACLE_ITERATIONS_SCENARIO0 = 776_640
-CPU_SAMPLE_ITERATIONS = 100_000
-# host cores the baseline uses (the GPU box's CPU share for one GPU)
-CPU_CORES = 16
+def generate(fn, ks):
+    """Build instances on host threads (the generators are C++ and release
+    the GIL)."""
+    with ThreadPoolExecutor(min(CPU_CORES, os.cpu_count() or 1)) as ex:
+        return list(ex.map(fn, ks))
 
 
-def cpu_baseline(problem: prima.LpProblem):
-    """The oracle (C restatement of Prima's FP64 CPU path, one thread per LP)
-    on scenario 0, one copy per core as PrimaZio.solveAll runs one LP per
-    available processor (PrimaZio.scala:61-66): time per iteration over a
-    bounded prefix on every core at once, scaled to LPs/s."""
-    from concurrent.futures import ThreadPoolExecutor
-    from oracle.oracle import OracleLp, default_params
-    cores = max(1, min(CPU_CORES, os.cpu_count() or 1))
-    copies = [OracleLp.from_csr(problem.n, problem.m, problem.n_eq, problem.row_ptr, problem.col,
-                                problem.val, problem.c, problem.q, problem.l, problem.u,
-                                problem.offset) for _ in range(cores)]
-
-    def run(o):  # the oracle's C solve releases the GIL
-        t = time.perf_counter()
-        s = o.solve(default_params(max_iterations=CPU_SAMPLE_ITERATIONS))
-        return time.perf_counter() - t, s.iterations
-
-    with ThreadPoolExecutor(cores) as ex:
-        runs = list(ex.map(run, copies))
-    per_lp = [dt / its * ORACLE_ITERATIONS_SCENARIO0 for dt, its in runs]
-    value = sum(1.0 / x for x in per_lp)
-    return dict(value=value, unit="LPs/s", cores=cores, kind="port",
-                sample=f"scenario 0 on the C restatement of Prima's ScalaKernels+Pdhg path, one "
-                       f"copy per core on {cores} cores at once: first {runs[0][1]} of the "
-                       f"{ORACLE_ITERATIONS_SCENARIO0} iterations it needs to reach 1e-8 "
-                       f"({max(dt for dt, _ in runs):.1f} s), scaled to one LP per core "
-                       f"({np.mean(per_lp):.0f} s each)",
-                cpu=cpu_model())
-
-
-SCHEDULES = {0: "auto", 1: "vector-1", 2: "vector-2", 3: "vector-4", 4: "vector-8",
-             5: "vector-16", 6: "vector-32", 7: "vector-64", 8: "row-blocks", 9: "panels"}
-
-
-def spmv_roofline(reps: int = 20, copy_gbs: float | None = None):
-    """The hot SpMV at the large-LP scale: A of the 2M-variable, 1M-row random
-    LP (lognormal row lengths, sigma 1, mean about 20, capped at 2048). The
-    autotuner times every schedule; the best is re-timed with HIP events.
-    Algorithmic bytes per SpMV: Z(s+4) + 4(R+1) + s*C + s*R."""
-    import math
-    p = prima.random_known(2, 2_000_000, 1_000_000, row_len=0, mu=math.log(20.0) - 0.5,
-                           sigma=1.0, max_len=2048)
-    R, Cn, Z = p.m, p.n, p.nnz
-    out = {"matrix": f"{R} x {Cn}, {Z} nnz, row lengths lognormal(ln 20 - 0.5, 1) <= 2048"}
-    rng = np.random.default_rng(0)
-    xh = rng.standard_normal(Cn)
-    for prec in (32, 64):
-        k = prima.HipKernels(0, prec)
-        A = k.uploadMatrix(R, Cn, p.row_ptr, p.col, p.val)
-        x, y = k.upload(xh), k.allocate(R)
-        times = {}
-        for sch in range(1, 10):
-            k.set_schedule(A, sch)
-            k.spmv(A, x, y)
-            k.download(y)
-            t = time.perf_counter()
-            for _ in range(reps):
-                k.spmv(A, x, y)
-            k.download(y)
-            times[SCHEDULES[sch]] = (time.perf_counter() - t) / reps * 1e3
-        choice, best_ms = k.autotune(A, reps)
-        sb = 4 if prec == 32 else 8
-        bytes_ = Z * (sb + 4) + 4 * (R + 1) + sb * Cn + sb * R
-        ref = p.to_scipy() @ xh
-        k.set_schedule(A, choice)
-        k.spmv(A, x, y)
-        got = k.download(y)
-        err = float(np.max(np.abs(got - ref) / (np.abs(p.to_scipy()) @ np.abs(xh) + 1e-30)))
-        out[f"f{prec}"] = {"schedule": SCHEDULES[choice], "ms": best_ms,
-                           "achieved_GBs": bytes_ / (best_ms * 1e-3) / 1e9,
-                           "frac_of_8TBs": bytes_ / (best_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
-                           "frac_of_copy": (bytes_ / (best_ms * 1e-3) / 1e9 / copy_gbs
-                                            if copy_gbs else None),
-                           "algorithmic_bytes": bytes_, "max_rel_err_vs_f64_host": err,
-                           "wall_ms_per_schedule": {k_: round(v, 3) for k_, v in times.items()}}
-        log(f"spmv f{prec}: {out[f'f{prec}']}")
-        del A, x, y
-        k.close()
-    return out
-
-
-def run_watched(k, fn, what):
+def run_watched(k, fn, what, every=20.0):
     """Run fn() in a worker thread; print the library's progress counters
-    (launch boundaries, accepted iterations, finished LPs) while it runs."""
-    import threading
+    while it runs."""
     out = {}
 
     def work():
@@ -198,14 +107,60 @@ def run_watched(k, fn, what):
     th.start()
     t = time.perf_counter()
     while th.is_alive():
-        th.join(20.0)
+        th.join(every)
         if th.is_alive():
             pr = k.progress()
             log(f"{what}: {time.perf_counter() - t:.0f} s, {pr['launches']} launches, "
-                f"{pr['accepted_iterations']} iterations, {pr['finished_lps']} LPs finished")
+                f"{pr['accepted_iterations']} iterations, {pr['finished_lps']} passes finished")
     if "e" in out:
         raise out["e"]
     return out["v"]
+
+
+def check(sols, labels, gold, what):
+    worst = 0.0
+    for k, s in zip(labels, sols):
+        if s.status != "Optimal":
+            raise SystemExit(f"{what}: LP {k} ended {s.status}")
+        if k in gold:
+            g = rel_gap(s.objective_value, gold[k])
+            worst = max(worst, g)
+            if g > 1e-8:
+                raise SystemExit(f"{what}: LP {k} objective {s.objective_value!r} vs HiGHS "
+                                 f"{gold[k]!r} (gap {g:.2e})")
+    return worst
+
+
+def dominant(stats):
+    """The kernel instantiation (float32 or float64 iterations) that took the
+    most device time, with its launches, time and algorithmic bytes."""
+    ms = stats["kernel_ms_by_precision"]
+    i = 1 if ms[1] >= ms[0] else 0
+    return dict(precision=64 if i else 32, launches=stats["launches_by_precision"][i],
+                kernel_ms=ms[i], algorithmic_bytes=stats["algorithmic_bytes_by_precision"][i])
+
+
+def roofline(stats, copy=None):
+    d = dominant(stats)
+    launches = max(1, d["launches"])
+    per_launch = d["algorithmic_bytes"] / launches
+    avg_ms = d["kernel_ms"] / launches
+    achieved = per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    ratio = TRAFFIC_PER_ALGORITHMIC_BYTE.get(d["precision"])
+    out = {"bound": "hbm",
+           "kernel": f"pdhg_persistent<{'double' if d['precision'] == 64 else 'float'}>",
+           "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+           "frac": achieved / PEAK_HBM_GBS,
+           "traffic": ratio * per_launch if ratio else None,
+           "algorithmic_bytes_per_launch": per_launch, "avg_launch_ms": avg_ms,
+           "launches": d["launches"],
+           "all_launches": {"kernel_ms": stats["kernel_ms"], "launches": stats["launches"],
+                            "achieved_GBs": (stats["algorithmic_bytes"] / (stats["kernel_ms"] * 1e-3)
+                                             / 1e9 if stats["kernel_ms"] else 0.0)}}
+    if copy:
+        out["achievable_copy"] = copy
+        out["frac_of_copy"] = achieved / copy["GBs"]
+    return out
 
 
 def copy_calibration(k):
@@ -222,13 +177,10 @@ def copy_calibration(k):
 
 
 def copy_bandwidth(k, reps: int = 20) -> dict:
-    """The achievable-copy figure beside the 8 TB/s peak, over 2^26 doubles
-    (512 MiB read and 512 MiB written): the seam's copy op (a device-to-device
-    hipMemcpyAsync) and its scale op (the library's own streaming kernel,
-    out = a*x), each wall-clocked over back-to-back launches between two
-    synchronising one-element dot products. The faster is the figure. (torch
-    is not used here: its bundled HIP runtime cannot share a process with
-    the library's.)"""
+    """Achievable copy over 2^26 doubles (512 MiB read and 512 MiB written):
+    the seam's copy op (a device-to-device hipMemcpyAsync) and its scale op
+    (the library's own streaming kernel), each wall-clocked over back-to-back
+    launches between two synchronising one-element dot products."""
     n = 1 << 26
     a = k.upload(np.ones(n))
     b = k.allocate(n)
@@ -249,42 +201,295 @@ def copy_bandwidth(k, reps: int = 20) -> dict:
     return {"GBs": max(memcpy, scale), "hipMemcpy_d2d_GBs": memcpy, "scale_kernel_GBs": scale}
 
 
-def cpu_model() -> str:
+# ---------------------------------------------------------------------------
+# the uneven scenario sweep (the headline)
+
+def sweep_goldens() -> dict:
+    with open(os.path.join(GOLDEN_DIR, "highs_sweep.json")) as f:
+        return {d["args"]["k"]: d["objective"] for d in json.load(f) if d["status"] == 0}
+
+
+def stratified(sizes, count):
+    """Indices at the midpoints of `count` equal-count size strata."""
+    order = sorted(range(len(sizes)), key=lambda i: (sizes[i], i))
+    return [order[int((j + 0.5) * len(order) / count)] for j in range(count)]
+
+
+def cpu_baseline(batch, pick, fp64_iterations, seconds=12.0):
+    """The oracle (C restatement of Prima's FP64 CPU path: ScalaKernels +
+    Pdhg + Scaling + Kkt, -O3, no fast-math), one LP per core as
+    PrimaZio.solveAll runs one LP per available processor (PrimaZio.scala:
+    61-66): each sampled LP on CPU_CORES / len(pick) cores at once, each copy
+    for about `seconds` of iterations. A whole sweep LP takes the CPU minutes
+    to hours, so the figure is extrapolated: seconds per iteration on a loaded
+    host times the iterations the same LP took in the device's FP64 solve."""
+    from oracle.oracle import OracleLp, default_params
+    cores = max(1, min(CPU_CORES, os.cpu_count() or 1))
+    per = max(1, cores // len(pick))
+    jobs = []
+    for i in pick:
+        p = batch[i]
+        for _ in range(per):
+            jobs.append((i, OracleLp.from_csr(p.n, p.m, p.n_eq, p.row_ptr, p.col, p.val, p.c,
+                                              p.q, p.l, p.u, p.offset)))
+    # a short probe sizes each copy's iteration count to about `seconds`
+    probe = {}
+    for i in pick:
+        o = next(o for j, o in jobs if j == i)
+        t = time.perf_counter()
+        o.solve(default_params(max_iterations=64, eps_abs=1e-30, eps_rel=1e-30))
+        probe[i] = max((time.perf_counter() - t) / 64, 1e-7)
+
+    def run(job):
+        i, o = job
+        its = max(64, int(seconds / probe[i]) // 64 * 64)
+        t = time.perf_counter()
+        s = o.solve(default_params(max_iterations=its, eps_abs=1e-30, eps_rel=1e-30))
+        return i, (time.perf_counter() - t) / max(1, s.iterations)
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(len(jobs)) as ex:  # the oracle's C solve releases the GIL
+        runs = list(ex.map(run, jobs))
+    wall = time.perf_counter() - t0
+    sec_per_it = {i: float(np.mean([r for j, r in runs if j == i])) for i in pick}
+    per_lp = [sec_per_it[i] * fp64_iterations[i] for i in pick]
+    value = cores / float(np.mean(per_lp))
+    return dict(value=value, unit="LPs/s", cores=cores, kind="port",
+                sample=(f"the denominator's {len(pick)} sampled LPs ({', '.join(str(i) for i in pick)}), "
+                        f"{per} copies each on {len(jobs)} threads at once for {wall:.0f} s; "
+                        f"extrapolated: seconds per iteration x the device FP64 iteration count "
+                        f"of the same LP (mean {np.mean(per_lp):.0f} s per LP per core), one LP "
+                        f"per core"),
+                seconds_per_lp=dict(zip([str(i) for i in pick], per_lp)), cpu=cpu_model())
+
+
+def headline(args, batch, gold, workload, sample_by_size):
+    """Mixed-precision batched steps until the budget, then plain FP64
+    batched, then plain FP64 one LP at a time on a stratified sample (the
+    metric's denominator), then the CPU baseline; one JSON line."""
+    t_all = time.perf_counter()
+    B = len(batch)
+    params = prima.PdhgParams(max_iterations=args.max_iterations)
+    mixed = prima.MixedParams(reduced_tolerance=args.reduced_tolerance,
+                              reduced_max_iterations=args.reduced_max_iterations,
+                              stall_iterations=args.stall_iterations)
+    k32 = prima.HipKernels(0, 32)
+    k64 = prima.HipKernels(0, 64)
     try:
-        with open("/proc/cpuinfo") as f:
-            for ln in f:
-                if ln.startswith("model name"):
-                    return ln.split(":", 1)[1].strip()
-    except OSError:
-        pass
-    return platform.processor() or platform.machine()
+        copy_calibration(k64)  # scales FETCH_SIZE / WRITE_SIZE in a counter run
+        warm = 0
+        for _ in range(min(args.warmup, 1)):  # no JIT: one small solve per context
+            for k in (k32, k64):
+                s = k.solve(prima.config0(), prima.PdhgParams(eps_abs=1e-4, eps_rel=1e-4))
+                if s.status != "Optimal":
+                    raise SystemExit(f"warm-up solve ended {s.status}")
+            warm += 1
+        k32.reset_stats()
+        steps, elapsed, worst, sols, first = 0, 0.0, 0.0, None, None
+        while steps < max(1, args.steps):
+            t = time.perf_counter()
+            sols = run_watched(k32, lambda: k32.solve_all(batch, params, mixed=mixed), "step")
+            dt = time.perf_counter() - t
+            worst = max(worst, check(sols, range(B), gold, "mixed batched"))
+            if first is None:
+                first = sols
+            elif any(a.objective_value != b.objective_value or a.iterations != b.iterations
+                     for a, b in zip(first, sols)):
+                raise SystemExit("mixed batched: a repeated step gave different results")
+            steps += 1
+            elapsed += dt
+            log(f"step {steps}: {dt:.1f} s, worst gap vs HiGHS {worst:.2e}")
+            if elapsed + dt > args.budget:
+                break
+        st = k32.stats()
+        value = B * steps / elapsed
+        copy = copy_bandwidth(k64)
+        its = [(s.mixed["reduced_iterations"], s.mixed["refinement_iterations"],
+                s.mixed["fallback_iterations"]) for s in sols]
+        heavy = sorted(range(B), key=lambda i: -sols[i].total_time_ms)[:6]
+        log("slowest LPs (n; f32/f64/cold f64 iterations; seconds): " + "; ".join(
+            f"#{i} {batch[i].n}; {its[i][0]}/{its[i][1]}/{its[i][2]}; "
+            f"{sols[i].total_time_ms / 1e3:.0f}" for i in heavy))
+
+        # plain FP64, the whole batch in one call
+        fp64_batched, s64b = None, None
+        if not args.no_fp64_batched:
+            k64.reset_stats()
+            t = time.perf_counter()
+            s64b = run_watched(k64, lambda: k64.solve_all(batch, params), "fp64 batched")
+            tb = time.perf_counter() - t
+            check(s64b, range(B), gold, "fp64 batched")
+            fp64_batched = {"value": B / tb, "unit": "LPs/s", "seconds": tb,
+                            "iterations_mean": float(np.mean([s.iterations for s in s64b])),
+                            "iterations_max": int(max(s.iterations for s in s64b)),
+                            "roofline": roofline(k64.stats())}
+            log(f"fp64 batched: {fp64_batched}")
+
+        # plain FP64 one LP at a time (the denominator) on a stratified
+        # sample: by size, or, for LPs of one size, by their FP64 iteration
+        # counts from the batched run
+        key = ([p.n for p in batch] if sample_by_size or s64b is None
+               else [s.iterations for s in s64b])
+        pick = stratified(key, args.unbatched)
+        k64.reset_stats()
+        times, s64 = {}, {}
+        for i in pick:
+            t = time.perf_counter()
+            s64[i] = k64.solve(batch[i], params)
+            times[i] = time.perf_counter() - t
+        t64 = sum(times.values())
+        check([s64[i] for i in pick], pick, gold, "fp64 unbatched")
+        fp64 = {"value": len(pick) / t64, "unit": "LPs/s", "seconds": t64,
+                "sample": (f"LPs {pick}: midpoints of {len(pick)} equal-count strata by "
+                           f"{'size' if sample_by_size or s64b is None else 'FP64 iteration count'}"),
+                "iterations": [s64[i].iterations for i in pick],
+                "seconds_each": [round(times[i], 3) for i in pick]}
+        if s64b is not None:
+            # every LP's batched FP64 iteration count times the sample's
+            # seconds per iteration alone: the whole batch one at a time
+            rate = t64 / sum(s64[i].iterations for i in pick)
+            fp64["whole_batch_estimate"] = B / (rate * sum(s.iterations for s in s64b))
+        log(f"fp64 unbatched sample: {fp64}")
+        line = {
+            "metric": METRIC, "value": value, "unit": "LPs/s", "n_gpus": 1, "steps": steps,
+            "warmup": warm, "ms_per_step": 1e3 * elapsed / steps, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32 iterations, f64 final pass",
+            "data": "synthetic",
+            "config": dict(workload, tolerance=1e-8,
+                           mixed={"reduced_tolerance": mixed.reduced_tolerance,
+                                  "reduced_max_iterations": mixed.reduced_max_iterations,
+                                  "stall_iterations": mixed.stall_iterations},
+                           max_iterations=params.max_iterations),
+            "fp64_unbatched": fp64,
+            "speedup_vs_fp64_unbatched": value / fp64["value"],
+            "fp64_batched": fp64_batched,
+            "speedup_vs_fp64_batched": value / fp64_batched["value"] if fp64_batched else None,
+            "worst_rel_gap_vs_highs": worst,
+            "goldens_checked": sum(1 for i in range(B) if i in gold),
+            "iterations_mean": [float(np.mean([a for a, _, _ in its])),
+                                float(np.mean([b for _, b, _ in its])),
+                                float(np.mean([c for _, _, c in its]))],
+            "cold_fallbacks": int(sum(1 for _, _, c in its if c > 0)),
+            "device_lps": B * steps / (st["kernel_ms"] * 1e-3) if st["kernel_ms"] else None,
+            "roofline": roofline(st, copy),
+        }
+        if not args.no_cpu:
+            log("timing the CPU baseline (oracle)")
+            line["cpu_baseline"] = cpu_baseline(batch, pick, {i: s64[i].iterations for i in pick})
+        line["wall_seconds"] = time.perf_counter() - t_all
+        print(json.dumps(line), flush=True)
+    finally:
+        k32.close()
+        k64.close()
+
+
+def sweep(args):
+    """The uneven scenario sweep (prima.config3, k = 0..--batch-1)."""
+    t = time.perf_counter()
+    batch = generate(prima.config3, range(args.batch))
+    n_tot, m_tot, z_tot = (sum(p.n for p in batch), sum(p.m for p in batch),
+                           sum(p.nnz for p in batch))
+    log(f"generated {len(batch)} sweep LPs in {time.perf_counter() - t:.1f} s: {n_tot} vars, "
+        f"{m_tot} rows, {z_tot} nnz")
+    headline(args, batch, sweep_goldens(),
+             {"workload": f"uneven scenario sweep: {len(batch)} dispatch LPs, buses "
+                          f"log-uniform in [50, 2000], snapshots in {{24, 72, 168}} "
+                          f"({n_tot} vars, {m_tot} rows, {z_tot} nnz)", "lps": len(batch)},
+             sample_by_size=True)
+
+
+def scenarios(args):
+    """Demand scenarios of the 200-bus, 24-snapshot dispatch LP (the config-1
+    LP: 34,800 vars, 14,400 rows, 64,750 nnz), per-bus demand x U[0.8, 1.2]
+    with seed 1000 + k; every objective checked against HiGHS
+    (tests/golden/highs_dispatch.json)."""
+    with open(os.path.join(GOLDEN_DIR, "highs_dispatch.json")) as f:
+        gold = {d["args"]["scenario_seed"] - 1000: d["objective"] for d in json.load(f)
+                if d["args"].get("seed") == 1 and d["args"].get("perturb") == 0.2
+                and "buses" not in d["args"]}
+    batch = generate(lambda k: prima.dispatch(seed=1, perturb=0.2, scenario_seed=1000 + k),
+                     range(args.batch))
+    p = batch[0]
+    headline(args, batch, gold,
+             {"workload": f"demand scenarios of the 200-bus, 24-snapshot dispatch LP "
+                          f"({p.n} vars, {p.m} rows, {p.nnz} nnz each)", "lps": len(batch)},
+             sample_by_size=False)
+
+
+# ---------------------------------------------------------------------------
+# other workloads
+
+def uc_frontier(args):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import milp_fixtures as mf
+    root, g, node = mf.uc_frontier()
+    B = min(args.batch, 512)
+    nodes = [node(k) for k in range(B)]
+    P = prima.PdhgParams(max_iterations=args.max_iterations)
+    mixed = prima.MixedParams(reduced_tolerance=args.reduced_tolerance,
+                              reduced_max_iterations=args.reduced_max_iterations,
+                              stall_iterations=args.stall_iterations)
+    with prima.HipKernels(0, 64) as k64, prima.HipKernels(0, 32) as k32:
+        t = time.perf_counter()
+        r = k64.solve(root, P)
+        root_s = time.perf_counter() - t
+        if r.status != "Optimal" or rel_gap(r.objective_value, g["root_objective"]) > 1e-8:
+            raise SystemExit(f"root relaxation: {r.status} {r.objective_value!r}")
+        log(f"root relaxation: {r.iterations} iterations, {root_s:.2f} s")
+        warm = [prima.WarmStart.of(r)] * B
+        k32.reset_stats()
+        t0 = time.perf_counter()
+        sols = run_watched(k32, lambda: k32.solve_all(nodes, P, mixed=mixed, warms=warm),
+                           "frontier")
+        elapsed = time.perf_counter() - t0
+        worst, bad = 0.0, []
+        for k, s in enumerate(sols):
+            gold = g["nodes"][k]["objective"]
+            if s.status != "Optimal" or rel_gap(s.objective_value, gold) > 1e-8:
+                bad.append((k, s.status, s.objective_value, gold))
+            else:
+                worst = max(worst, rel_gap(s.objective_value, gold))
+        if bad:
+            raise SystemExit(f"frontier nodes off: {bad[:8]}")
+        st = k32.stats()
+    print(json.dumps({
+        "metric": "node LPs/sec at 1e-8 rel. KKT, batched B&B frontier, 1 MI355X",
+        "value": B / elapsed, "unit": "LPs/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": 1e3 * elapsed, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 iterations, f64 final pass", "data": "synthetic",
+        "config": {"workload": "unit-commitment B&B frontier: 100 buses x 24 snapshots "
+                               "(37,200 vars, 49,200 rows, 154,800 nnz), depth-9 frontier, "
+                               "warm-started from the root", "batch": B, "tolerance": 1e-8},
+        "root_iterations": r.iterations, "root_seconds": root_s,
+        "iterations_mean": [float(np.mean([s.mixed["reduced_iterations"] for s in sols])),
+                            float(np.mean([s.mixed["refinement_iterations"] for s in sols])),
+                            float(np.mean([s.mixed["fallback_iterations"] for s in sols]))],
+        "iterations_max": int(max(s.iterations for s in sols)),
+        "worst_rel_gap_vs_highs": worst, "roofline": roofline(st)}), flush=True)
 
 
 def kernel_profile(args):
     B = args.batch
-    batch = [scenario(k) for k in range(B)]
+    batch = generate(lambda k: prima.dispatch(seed=1, perturb=0.2, scenario_seed=1000 + k),
+                     range(B))
     out = {"workload": f"{B} dispatch LPs (200 buses x 24 snapshots), {args.iterations} "
                        f"iterations each, tolerance 1e-12"}
-    k = prima.HipKernels(0, 64)
-    out["copy"] = copy_calibration(k)
-    for prec in (64, 32):
-        kk = k if prec == 64 else prima.HipKernels(0, 32)
-        P = prima.PdhgParams(eps_abs=1e-12, eps_rel=1e-12, max_iterations=args.iterations,
-                             workgroups_per_lp=args.wg)
-        kk.reset_stats()
-        t = time.perf_counter()
-        sols = kk.solve_all(batch, P)
-        wall = time.perf_counter() - t
-        st = kk.stats()
-        its = sum(s.iterations for s in sols)
-        out[f"f{prec}"] = {
-            "launches": st["launches"], "kernel_ms": st["kernel_ms"],
-            "avg_launch_ms": st["kernel_ms"] / max(1, st["launches"]),
-            "algorithmic_bytes_per_launch": st["algorithmic_bytes"] / max(1, st["launches"]),
-            "achieved_GBs": st["algorithmic_bytes"] / (st["kernel_ms"] * 1e-3) / 1e9,
-            "lp_iterations_per_s": its / (st["kernel_ms"] * 1e-3), "wall_s": wall,
-            "workgroups_per_lp": sols[0].workgroups}
-        log(f"f{prec}: {out[f'f{prec}']}")
+    with prima.HipKernels(0, 64) as k64, prima.HipKernels(0, 32) as k32:
+        out["copy"] = copy_calibration(k64)
+        for prec, kk in ((64, k64), (32, k32)):
+            P = prima.PdhgParams(eps_abs=1e-12, eps_rel=1e-12, max_iterations=args.iterations,
+                                 workgroups_per_lp=args.wg)
+            kk.reset_stats()
+            t = time.perf_counter()
+            sols = kk.solve_all(batch, P)
+            wall = time.perf_counter() - t
+            st = kk.stats()
+            its = sum(s.iterations for s in sols)
+            out[f"f{prec}"] = dict(roofline(st), wall_s=wall,
+                                   lp_iterations_per_s=its / (st["kernel_ms"] * 1e-3),
+                                   workgroups_per_lp=sols[0].workgroups)
+            log(f"f{prec}: {out[f'f{prec}']}")
     print(json.dumps(out), flush=True)
 
 
@@ -294,96 +499,70 @@ def lopf_scigrid(args):
     and solved on the device in float64 and in mixed precision, against
     HiGHS's objective on the same LP."""
     from noaidi_amd import lopf, network
-    gold_dir = os.path.join(ROOT, "tests", "golden", "pypsa")
+    gold_dir = os.path.join(GOLDEN_DIR, "pypsa")
     with open(os.path.join(gold_dir, "highs.json")) as f:
         target = json.load(f)["scigrid-de"]["objective"]
-    t = time.perf_counter()
     net = network.read_folder(os.path.join(gold_dir, "scigrid-de"))
-    model = lopf.build(net)
-    build_s = time.perf_counter() - t
-    p = model.problem
+    p = lopf.build(net).problem
     out = {}
-    for tol in (1e-6, 1e-8):
-        P = prima.PdhgParams(eps_abs=tol, eps_rel=tol, max_iterations=4_000_000)
-        for label, prec, mixed in (("f64", 64, None),
-                                   ("mixed", 32, prima.MixedParams(reduced_tolerance=1e-6,
-                                                                   reduced_max_iterations=60_000))):
-            with prima.HipKernels(0, prec) as k:
-                t = time.perf_counter()
-                s = run_watched(k, (lambda: k.mixed_solve(p, P, mixed)) if mixed else
-                                (lambda: k.solve(p, P)), f"{label} {tol:g}")
-                el = time.perf_counter() - t
-            out[f"{label}_{tol:g}"] = {"status": s.status, "seconds": el,
-                                       "iterations": s.iterations,
-                                       "rel_gap_vs_highs": rel_gap(s.objective_value, target),
-                                       "workgroups": s.workgroups}
-            log(f"scigrid-de {label} {tol:g}: {out[f'{label}_{tol:g}']}")
-    line = {
-        "metric": "PyPSA LOPF solve time, scigrid-de, 1 MI355X",
-        "value": out["mixed_1e-08"]["seconds"], "unit": "s", "n_gpus": 1, "steps": 1,
-        "warmup": 0, "ms_per_step": 1e3 * out["mixed_1e-08"]["seconds"],
-        "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
-        "dtype": "f32 iterations + f64 refinement", "data": "PyPSA scigrid-de CSV export",
-        "config": {"workload": f"scigrid-de LOPF ({p.n} vars, {p.m} rows, {p.nnz} nnz, "
-                               f"{len(net.snapshots)} snapshots)", "tolerance": 1e-8},
-        "build_seconds": build_s, "solves": out,
-    }
-    print(json.dumps(line), flush=True)
+    for label, prec, mixed in (("f64", 64, None),
+                               ("mixed", 32, prima.MixedParams(reduced_tolerance=1e-6,
+                                                               reduced_max_iterations=60_000))):
+        with prima.HipKernels(0, prec) as k:
+            P = prima.PdhgParams(max_iterations=args.max_iterations)
+            t = time.perf_counter()
+            s = run_watched(k, (lambda: k.mixed_solve(p, P, mixed)) if mixed else
+                            (lambda: k.solve(p, P)), label)
+            el = time.perf_counter() - t
+        out[label] = {"status": s.status, "seconds": el, "iterations": s.iterations,
+                      "rel_gap_vs_highs": rel_gap(s.objective_value, target)}
+        log(f"scigrid-de {label}: {out[label]}")
+    print(json.dumps({"metric": "PyPSA LOPF solve time, scigrid-de, 1 MI355X",
+                      "value": out["mixed"]["seconds"], "unit": "s", "n_gpus": 1, "steps": 1,
+                      "warmup": 0, "ms_per_step": 1e3 * out["mixed"]["seconds"],
+                      "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
+                      "dtype": "f32 iterations, f64 final pass",
+                      "data": "PyPSA scigrid-de CSV export",
+                      "config": {"workload": f"scigrid-de LOPF ({p.n} vars, {p.m} rows, "
+                                             f"{p.nnz} nnz)", "tolerance": 1e-8},
+                      "solves": out}), flush=True)
 
 
-def uc_frontier(args):
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import milp_fixtures as mf
-    root, g, node = mf.uc_frontier()
-    B = min(args.batch, 512)
-    nodes = [node(k) for k in range(B)]
-    # the headline's caps and mixed-precision knobs: node iteration counts
-    # have the same heavy tail
-    P = prima.PdhgParams(max_iterations=4_000_000, workgroups_per_lp=args.wg)
-    mixed = prima.MixedParams(reduced_tolerance=args.reduced_tolerance,
-                              reduced_max_iterations=args.reduced_max_iterations,
-                              stall_iterations=1_000_000)
-    k64 = prima.HipKernels(0, 64)
-    k32 = prima.HipKernels(0, 32)
-    t = time.perf_counter()
-    r = k64.solve(root, P)
-    root_s = time.perf_counter() - t
-    if r.status != "Optimal" or rel_gap(r.objective_value, g["root_objective"]) > 1e-8:
-        raise SystemExit(f"root relaxation: {r.status} {r.objective_value!r}")
-    log(f"root relaxation: {r.iterations} iterations, {root_s:.2f} s")
-    warm = [prima.WarmStart.of(r)] * B
-    k32.reset_stats()
-    t0 = time.perf_counter()
-    sols = run_watched(k32, lambda: k32.solve_all(nodes, P, mixed=mixed, warms=warm),
-                       "frontier")
-    elapsed = time.perf_counter() - t0
-    worst = 0.0
-    for k, s in enumerate(sols):
-        gold = g["nodes"][k]["objective"]
-        if s.status != "Optimal" or rel_gap(s.objective_value, gold) > 1e-8:
-            raise SystemExit(f"node {k}: {s.status} {s.objective_value!r} vs HiGHS {gold!r}")
-        worst = max(worst, rel_gap(s.objective_value, gold))
-    st = k32.stats()
-    achieved = st["algorithmic_bytes"] / (st["kernel_ms"] * 1e-3) / 1e9 if st["kernel_ms"] else 0
-    launches = max(1, st["launches"])
-    line = {
-        "metric": "node LPs/sec at 1e-8 rel. KKT, batched B&B frontier, 1 MI355X",
-        "value": B / elapsed, "unit": "LPs/s", "n_gpus": 1, "steps": 1, "warmup": 0,
-        "ms_per_step": 1e3 * elapsed, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32 iterations + f64 refinement", "data": "synthetic",
-        "config": {"workload": "unit-commitment B&B frontier: 100 buses x 24 snapshots "
-                               "(37,200 vars, 49,200 rows, 154,800 nnz), depth-9 frontier",
-                   "batch": B, "tolerance": 1e-8},
-        "root_iterations": r.iterations, "root_seconds": root_s,
-        "iterations_mean": [float(np.mean([s.mixed["reduced_iterations"] for s in sols])),
-                            float(np.mean([s.mixed["refinement_iterations"] for s in sols]))],
-        "worst_rel_gap_vs_highs": worst,
-        "roofline": {"bound": "hbm", "kernel": "pdhg_persistent", "achieved": achieved,
-                     "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
-                     "traffic": None, "avg_launch_ms": st["kernel_ms"] / launches,
-                     "launches": st["launches"]},
-    }
-    print(json.dumps(line), flush=True)
+SCHEDULES = {0: "auto", 1: "vector-1", 2: "vector-2", 3: "vector-4", 4: "vector-8",
+             5: "vector-16", 6: "vector-32", 7: "vector-64", 8: "row-blocks", 9: "panels"}
+
+
+def spmv_roofline(args, reps: int = 20):
+    """The SpMV at the large-LP scale: A of the 2M-variable, 1M-row random LP
+    (lognormal row lengths, sigma 1, mean about 20, capped at 2048). The
+    autotuner times every schedule; algorithmic bytes per SpMV:
+    Z(s+4) + 4(R+1) + s*C + s*R."""
+    p = prima.random_known(2, 2_000_000, 1_000_000, row_len=0, mu=math.log(20.0) - 0.5,
+                           sigma=1.0, max_len=2048)
+    R, Cn, Z = p.m, p.n, p.nnz
+    out = {"matrix": f"{R} x {Cn}, {Z} nnz, row lengths lognormal(ln 20 - 0.5, 1) <= 2048"}
+    xh = np.random.default_rng(0).standard_normal(Cn)
+    with prima.HipKernels(0, 64) as kc:
+        out["achievable_copy"] = copy_bandwidth(kc)
+    for prec in (32, 64):
+        with prima.HipKernels(0, prec) as k:
+            A = k.uploadMatrix(R, Cn, p.row_ptr, p.col, p.val)
+            x, y = k.upload(xh), k.allocate(R)
+            choice, best_ms = k.autotune(A, reps)
+            sb = 4 if prec == 32 else 8
+            bytes_ = Z * (sb + 4) + 4 * (R + 1) + sb * Cn + sb * R
+            k.set_schedule(A, choice)
+            k.spmv(A, x, y)
+            got = k.download(y)
+            ref = p.to_scipy() @ xh
+            err = float(np.max(np.abs(got - ref) / (np.abs(p.to_scipy()) @ np.abs(xh) + 1e-30)))
+            out[f"f{prec}"] = {"schedule": SCHEDULES[choice], "ms": best_ms,
+                               "achieved_GBs": bytes_ / (best_ms * 1e-3) / 1e9,
+                               "frac_of_8TBs": bytes_ / (best_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                               "algorithmic_bytes": bytes_, "max_rel_err_vs_f64_host": err}
+            log(f"spmv f{prec}: {out[f'f{prec}']}")
+            del A, x, y
+    print(json.dumps(out), flush=True)
 
 
 def main():
@@ -391,185 +570,30 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--budget", type=float, default=150.0,
+                    help="seconds of timed steps after which no further step starts")
     ap.add_argument("--batch", type=int, default=256)
-    ap.add_argument("--wg", type=int, default=0, help="workgroups per LP (0: tuner)")
-    ap.add_argument("--unbatched", type=int, default=4,
-                    help="scenarios solved one at a time for the FP64 denominator")
+    ap.add_argument("--unbatched", type=int, default=8,
+                    help="strata sampled for the FP64 one-at-a-time denominator")
+    ap.add_argument("--wg", type=int, default=0, help="workgroups per LP (0: scheduler)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--fp64-batched", action="store_true",
-                    help="also solve the whole batch in plain FP64 (separates the batching "
-                         "gain from the precision gain; about as long as the mixed step)")
-    ap.add_argument("--no-spmv", action="store_true")
-    ap.add_argument("--reduced-tolerance", type=float, default=1e-6,
-                    help="tolerance of the float32 pass (MixedPrecision.Params.reducedTolerance)")
-    ap.add_argument("--reduced-max-iterations", type=int, default=60_000,
-                    help="iteration cap of the float32 pass")
-    ap.add_argument("--only-spmv", action="store_true")
-    ap.add_argument("--workload", default="dispatch",
-                    choices=["dispatch", "uc-frontier", "kernel", "lopf"])
+    ap.add_argument("--no-fp64-batched", action="store_true",
+                    help="skip solving the whole batch in plain FP64 batched")
+    ap.add_argument("--max-iterations", type=int, default=4_000_000)
+    ap.add_argument("--reduced-tolerance", type=float, default=1e-6)
+    ap.add_argument("--reduced-max-iterations", type=int, default=60_000)
+    ap.add_argument("--stall-iterations", type=int, default=1_000_000)
     ap.add_argument("--iterations", type=int, default=2048,
                     help="iterations per LP for --workload kernel")
+    ap.add_argument("--workload", default="scenarios",
+                    choices=["sweep", "scenarios", "uc-frontier", "kernel", "lopf", "spmv"])
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on exactly one GPU: this build solves independent LPs per GPU "
               "and has no multi-GPU path to measure", file=sys.stderr)
         sys.exit(2)
-
-    if args.only_spmv:
-        with prima.HipKernels(0, 64) as kc:
-            copy = copy_calibration(kc)
-            copy["achievable"] = copy_bandwidth(kc)
-        print(json.dumps({"copy": copy,
-                          "spmv_roofline": spmv_roofline(copy_gbs=copy["achievable"]["GBs"])}),
-              flush=True)
-        return
-    if args.workload == "uc-frontier":
-        uc_frontier(args)
-        return
-    if args.workload == "kernel":
-        kernel_profile(args)
-        return
-    if args.workload == "lopf":
-        lopf_scigrid(args)
-        return
-
-    B = args.batch
-    batch = [scenario(k) for k in range(B)]
-    gold = golden_objectives()
-    # Iteration counts of these scenarios have a heavy tail (66k to 1.4M
-    # float64 iterations to 1e-8, depending on the LP and on reduction order),
-    # so the cap is far above the reference's 100k default.
-    params = prima.PdhgParams(max_iterations=4_000_000, workgroups_per_lp=args.wg)
-    # the reference's two passes (MixedPrecision.Params knobs): the float32
-    # pass runs to 1e-6 or 60k iterations (defaults 1e-5, 20k), which gives
-    # the float64 pass a closer start and fewer stalls (256 scenarios: 84.6 s
-    # against 118.7 s at 1e-5 / 60k); a warm-started float64 pass that has not
-    # converged after 1M iterations restarts cold (stall_iterations)
-    mixed = prima.MixedParams(reduced_tolerance=args.reduced_tolerance,
-                              reduced_max_iterations=args.reduced_max_iterations,
-                              stall_iterations=1_000_000)
-    k32 = prima.HipKernels(0, 32)
-    k64 = prima.HipKernels(0, 64)
-    # the dispatch that scales FETCH_SIZE / WRITE_SIZE in a counter run of this command
-    copy_calibration(k64)
-
-    for _ in range(args.warmup):
-        t = time.perf_counter()
-        check(k32.solve_all(batch, params, mixed=mixed), range(B), gold, "warmup")
-        log(f"warmup step: {time.perf_counter() - t:.2f} s")
-
-    k32.reset_stats()
-    t0 = time.perf_counter()
-    worst = 0.0
-    iters = []
-    for _ in range(args.steps):
-        sols = run_watched(k32, lambda: k32.solve_all(batch, params, mixed=mixed), "step")
-        worst = max(worst, check(sols, range(B), gold, "mixed batched"))
-        iters += [(s.mixed["reduced_iterations"], s.mixed["refinement_iterations"],
-                   s.mixed["fallback_iterations"]) for s in sols]
-        log(f"step done: {time.perf_counter() - t0:.2f} s cumulative, worst gap {worst:.2e}")
-        heavy = sorted(range(B), key=lambda i: -(sols[i].mixed["refinement_iterations"]
-                                                + sols[i].mixed["fallback_iterations"]))[:8]
-        log("heaviest LPs (f32, f64, cold f64 iterations; seconds; final workgroups): " + "; ".join(
-            f"#{i} {sols[i].mixed['reduced_iterations']}/{sols[i].mixed['refinement_iterations']}"
-            f"/{sols[i].mixed['fallback_iterations']} {sols[i].total_time_ms / 1e3:.1f}s "
-            f"G{sols[i].workgroups}" for i in heavy))
-    elapsed = time.perf_counter() - t0
-    st = k32.stats()
-    value = B * args.steps / elapsed
-
-    # plain FP64 PDHG, one LP at a time (the denominator), on the first
-    # --unbatched scenarios of the same batch
-    fp64_batched = None
-    if args.fp64_batched:
-        k64.reset_stats()
-        t = time.perf_counter()
-        sb = run_watched(k64, lambda: k64.solve_all(batch, params), "fp64 batched")
-        tb = time.perf_counter() - t
-        check(sb, range(B), gold, "fp64 batched")
-        fp64_batched = {"value": B / tb, "seconds": tb,
-                        "iterations_mean": float(np.mean([s_.iterations for s_ in sb])),
-                        "iterations_max": int(max(s_.iterations for s_ in sb))}
-        log(f"fp64 batched {B / tb:.3f} LPs/s ({tb:.1f} s)")
-
-    U = min(args.unbatched, B)
-    k64.reset_stats()
-    t = time.perf_counter()
-    s64 = [k64.solve(batch[k], params) for k in range(U)]
-    t64 = time.perf_counter() - t
-    check(s64, range(U), gold, "fp64 unbatched")
-    fp64_lps = U / t64
-    st64 = k64.stats()
-    log(f"mixed batched {value:.3f} LPs/s; fp64 unbatched {fp64_lps:.3f} LPs/s "
-        f"({[s.iterations for s in s64]} iterations, {t64:.2f} s); kernel stats {st}")
-
-    achieved = st["algorithmic_bytes"] / (st["kernel_ms"] * 1e-3) / 1e9 if st["kernel_ms"] else 0
-    copy = copy_bandwidth(k64)
-    copy_gbs = copy["GBs"]
-    log(f"achievable copy: {copy}")
-    launches = max(1, st["launches"])
-    line = {
-        "metric": "LPs/sec at 1e-8 rel. KKT: mixed-prec batched vs plain FP64 PDHG, 1 MI355X",
-        "value": value,
-        "unit": "LPs/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": 1e3 * elapsed / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32 iterations + f64 refinement",
-        "data": "synthetic",
-        "config": {
-            "workload": "dispatch LP, 200 buses x 24 snapshots (34,800 vars, 14,400 rows, "
-                        "64,750 nnz), batch of demand scenarios",
-            "batch": B,
-            "n": batch[0].n, "m": batch[0].m, "nnz": batch[0].nnz,
-            "tolerance": 1e-8,
-            "mixed": {"reduced_tolerance": mixed.reduced_tolerance,
-                      "reduced_max_iterations": mixed.reduced_max_iterations,
-                      "stall_iterations": mixed.stall_iterations},
-        },
-        "device_lps": B * args.steps / (st["kernel_ms"] * 1e-3) if st["kernel_ms"] else None,
-        "fp64_unbatched_lps": fp64_lps,
-        "fp64_batched": fp64_batched,
-        "fp64_unbatched_sample": U,
-        "speedup_vs_fp64_unbatched": value / fp64_lps,
-        "fp64_unbatched_iterations": [s.iterations for s in s64],
-        "mixed_iterations_mean": [float(np.mean([a for a, _, _ in iters])),
-                                  float(np.mean([b for _, b, _ in iters]))],
-        "cold_fallbacks": int(sum(1 for _, _, c in iters if c > 0)),
-        "worst_rel_gap_vs_highs": worst,
-        "roofline": {
-            "bound": "hbm",
-            "kernel": "pdhg_persistent",
-            "achieved": achieved,
-            "peak": PEAK_HBM_GBS,
-            "unit": "GB/s",
-            "frac": achieved / PEAK_HBM_GBS,
-            "achievable_copy": copy,
-            "frac_of_copy": achieved / copy_gbs,
-            "traffic": TRAFFIC_PER_ALGORITHMIC_BYTE["mixed"] * st["algorithmic_bytes"] / launches,
-            "traffic_basis": "PMC FETCH_SIZE+WRITE_SIZE per algorithmic byte over the mixed "
-                             "step of this command (0.84: 137.9 GB per launch against 163.8), "
-                             "times this run's algorithmic bytes per launch",
-            "algorithmic_bytes_per_launch": st["algorithmic_bytes"] / launches,
-            "avg_launch_ms": st["kernel_ms"] / launches,
-            "launches": st["launches"],
-        },
-        "fp64_roofline": {
-            "achieved": (st64["algorithmic_bytes"] / (st64["kernel_ms"] * 1e-3) / 1e9
-                         if st64["kernel_ms"] else 0),
-            "unit": "GB/s",
-        },
-    }
-    if not args.no_spmv:
-        line["spmv_roofline"] = spmv_roofline(copy_gbs=copy_gbs)
-    if not args.no_cpu:
-        log("timing the CPU baseline (oracle, one LP per core)")
-        line["cpu_baseline"] = cpu_baseline(batch[0])
-    print(json.dumps(line), flush=True)
+    {"sweep": sweep, "scenarios": scenarios, "uc-frontier": uc_frontier,
+     "kernel": kernel_profile, "lopf": lopf_scigrid, "spmv": spmv_roofline}[args.workload](args)
 
 
 if __name__ == "__main__":

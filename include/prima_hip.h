@@ -347,7 +347,14 @@ typedef struct {
   int64_t trials;
   int64_t evaluations;
   double algorithmic_bytes; /* compulsory HBM bytes of the work done: every array
-                             * counted once, no gather amplification */
+                             * counted once, no gather amplification, and a
+                             * matrix that LPs of one call share counted once
+                             * per pass over it */
+  /* launches, duration and algorithmic bytes split by the kernel's iteration
+   * precision (index 0: float32, 1: float64) */
+  int64_t launches_by_precision[2];
+  double kernel_ms_by_precision[2];
+  double algorithmic_bytes_by_precision[2];
 } prima_stats;
 int prima_stats_get(const prima_ctx* ctx, prima_stats* out);
 int prima_stats_reset(prima_ctx* ctx);

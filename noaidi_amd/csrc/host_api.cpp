@@ -802,17 +802,42 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
   // Compulsory bytes of a pass (each array counted once, no gather
   // amplification): accepted iteration 2Z(s+4) + 4(m+n+2) + s(13n+11m);
   // rejected trial Z(s+4) + 4(m+1) + s(n+m) + s(7n+5m); checkpoint
-  // Z(s+4) + 4(m+1) + s(n+m) + 8(4n+4m).
-  auto account = [&](const SolveJob& J, const SolveState& st) {
-    const double Z = (double)J.h.nnz, n = J.h.n, m = J.h.m, sb = (double)s;
+  // Z(s+4) + 4(m+1) + s(n+m) + 8(4n+4m). The matrix terms (values, columns,
+  // row pointers of A and of its transpose) are charged once per pass over a
+  // matrix that LPs of the wave share: for the LP of the sharing set with the
+  // most passes (charge_matrices, at the end of the wave).
+  std::vector<double> a_passes(idx.size(), 0.0), t_passes(idx.size(), 0.0);
+  auto account = [&](size_t k, const SolveState& st) {
+    const SolveJob& J = jobs[idx[k]];
+    const double n = J.h.n, m = J.h.m, sb = (double)s;
     const double it = st.iteration, rej = (double)(st.total_trials - st.iteration);
-    const double b_iter = 2 * Z * (sb + 4) + 4 * (m + n + 2) + sb * (13 * n + 11 * m);
-    const double b_rej = Z * (sb + 4) + 4 * (m + 1) + sb * (n + m) + sb * (7 * n + 5 * m);
-    const double b_eval = Z * (sb + 4) + 4 * (m + 1) + sb * (n + m) + 8 * (4 * n + 4 * m);
-    ctx->stats.algorithmic_bytes += it * b_iter + rej * b_rej + st.evals * b_eval;
+    const double v_iter = sb * (13 * n + 11 * m);
+    const double v_rej = sb * (n + m) + sb * (7 * n + 5 * m);
+    const double v_eval = sb * (n + m) + 8 * (4 * n + 4 * m);
+    const double bytes = it * v_iter + rej * v_rej + st.evals * v_eval;
+    ctx->stats.algorithmic_bytes += bytes;
+    ctx->stats.algorithmic_bytes_by_precision[prec == 64] += bytes;
     ctx->stats.accepted_iterations += st.iteration;
     ctx->stats.trials += st.total_trials;
     ctx->stats.evaluations += st.evals;
+    a_passes[k] += it + rej + st.evals;
+    t_passes[k] += it;
+  };
+  auto charge_matrices = [&]() {
+    for (size_t k = 0; k < idx.size(); ++k) {
+      if (owner_of[k] >= 0) continue;  // charged with its owner
+      double a = a_passes[k], t = t_passes[k];
+      for (size_t e = k + 1; e < idx.size(); ++e)
+        if (owner_of[e] == (int)k) {
+          a = std::max(a, a_passes[e]);
+          t = std::max(t, t_passes[e]);
+        }
+      const HostLp& h = jobs[idx[k]].h;
+      const double Z = (double)h.nnz, sb = (double)s;
+      const double bytes = a * (Z * (sb + 4) + 4.0 * (h.m + 1)) + t * (Z * (sb + 4) + 4.0 * (h.n + 1));
+      ctx->stats.algorithmic_bytes += bytes;
+      ctx->stats.algorithmic_bytes_by_precision[prec == 64] += bytes;
+    }
   };
 
   // Starting point (Pdhg.scala:265-303): x0 = proj(warm / colScale or 0),
@@ -1181,6 +1206,8 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
       HIP_TRY(ctx, hipEventElapsedTime(&ev_ms, ctx->ev0, ctx->ev1));
       ctx->stats.kernel_ms += ev_ms;
       ctx->stats.launches += 1;
+      ctx->stats.kernel_ms_by_precision[prec == 64] += ev_ms;
+      ctx->stats.launches_by_precision[prec == 64] += 1;
     }
     {
       const int rc = check_abort();
@@ -1248,7 +1275,7 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
         if (states[k].status != PRIMA_ITERATION_LIMIT || J.cold_budget <= 0 ||
             J.capped_iterations >= 0 || J.shadow >= 0)
           continue;
-        account(J, states[k]);
+        account(k, states[k]);
         ctx->prog_base += states[k].iteration;
         J.capped_iterations = states[k].iteration;
         HIP_TRY(ctx, start_point(k, nullptr));
@@ -1295,6 +1322,8 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
           HIP_TRY(ctx, hipEventElapsedTime(&ev_ms, ctx->ev0, ctx->ev1));
           ctx->stats.kernel_ms += ev_ms;
           ctx->stats.launches += 1;
+          ctx->stats.kernel_ms_by_precision[prec == 64] += ev_ms;
+          ctx->stats.launches_by_precision[prec == 64] += 1;
         }
         {
           const int rc = check_abort();
@@ -1336,7 +1365,7 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
     // whose warm pass still stands (an interrupted race)
     const bool twin_unneeded = J.shadow_of >= 0 && !dropped[(size_t)slot_of[J.shadow_of]];
     if (dropped[k] || dormant[k] || twin_unneeded) {
-      if (!dormant[k]) account(J, st);
+      if (!dormant[k]) account(k, st);
       continue;
     }
     if (st.status < 0) return fail(ctx, PRIMA_EDEVICE, "solve ended without a status");
@@ -1349,7 +1378,7 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
     if (J.y_out && J.h.m)
       HIP_TRY(ctx, hipMemcpy(J.y_out, W.arena + L.y_out, (size_t)J.h.m * 8,
                              hipMemcpyDeviceToHost));
-    account(J, st);
+    account(k, st);
     prima_result* r = J.res;
     r->status = st.status;
     r->iterations = st.iteration;
@@ -1369,6 +1398,7 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
     r->evaluations = st.evals;
     r->workgroups = J.h.G;
   }
+  charge_matrices();
   return PRIMA_OK;
 }
 

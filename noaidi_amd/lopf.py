@@ -83,8 +83,8 @@ def _library(net: Network, component: str) -> Dict[str, Table]:
 
 
 def _unknown(component, entity, name, known):
-    return UnknownType(f"{component} '{entity}' has type '{name}', which is in neither the "
-                       f"network's own types nor the {len(known)} shipped ones")
+    return UnknownType(f"{component} '{entity}': type '{name}' is not defined by the network "
+                       f"and is not one of the {len(known)} standard types shipped here")
 
 
 def expand_standard_types(net: Network) -> Network:
@@ -111,8 +111,8 @@ def expand_standard_types(net: Network) -> Network:
             t = lib[name]
             length, par = lines.float("length", e), lines.float("num_parallel", e)
             if not par > 0.0:
-                raise UnknownType(f"Line '{e}' has num_parallel = {par}, which its type's "
-                                  "impedance is divided by")
+                raise UnknownType(f"Line '{e}': num_parallel = {par} cannot divide the "
+                                  "impedance of its line type")
             r.append(t.float("r_per_length", name) * length / par)
             x.append(t.float("x_per_length", name) * length / par)
             b.append(2.0 * math.pi * 1e-9 * t.float("f_nom", name) * t.float("c_per_length", name)
@@ -255,11 +255,11 @@ def impedance(net: Network, component: str, entity: str, carrier: str) -> float:
     if component == "Transformer":
         s_nom = table.float("s_nom", entity)
         if not s_nom > 0.0:
-            raise UnsupportedNetwork(f"Transformer '{entity}' has s_nom = {s_nom}, which is its "
-                                     "per-unit base")
+            raise UnsupportedNetwork(f"Transformer '{entity}': s_nom = {s_nom} cannot serve as "
+                                     "the base of its per-unit impedance")
         return raw * tap_ratio(table, entity) / s_nom
-    raise UnsupportedNetwork(f"{component} '{entity}' is a passive branch whose per-unit base is "
-                             "not known here")
+    raise UnsupportedNetwork(f"{component} '{entity}': no per-unit impedance base is defined "
+                             "for this kind of passive branch")
 
 
 # --------------------------------------------------------------------------
@@ -327,8 +327,8 @@ def _reject_expansion(net: Network):
                                          "annuitises; only capital_cost is modelled")
             mod = f"{attr}_mod"
             if mod in t.static and t.float(mod, e) != 0.0:
-                raise UnsupportedNetwork(f"{component} '{e}' sets {mod}, so its capacity is an "
-                                         "integer number of blocks")
+                raise UnsupportedNetwork(f"{component} '{e}': {mod} makes the capacity a whole "
+                                         "number of modules, which needs integer variables")
             lo, hi = t.float(f"{attr}_min", e), t.float(f"{attr}_max", e)
             if lo > hi:
                 raise UnsupportedNetwork(f"{component} '{e}' has {attr}_min = {lo} above "
@@ -340,7 +340,8 @@ def _reject_expansion(net: Network):
     if g is not None and "committable" in g.static:
         for e in g.ids:
             if g.bool("committable", e) and is_extendable(g, e):
-                raise UnsupportedNetwork(f"Generator '{e}' is both committable and extendable")
+                raise UnsupportedNetwork(f"Generator '{e}': a committable unit cannot also have "
+                                         "an extendable capacity here")
 
 
 # --------------------------------------------------------------------------
@@ -371,8 +372,8 @@ def _quadratic(table: Table) -> List[str]:
 def _reject_standing_loss(table: Table, component: str):
     def check(e, v, where=""):
         if not (v >= 0.0 and v < 1.0):
-            raise UnsupportedNetwork(f"{component} '{e}' has standing_loss = {v}{where}, which is "
-                                     "not a fraction below 1")
+            raise UnsupportedNetwork(f"{component} '{e}': standing_loss = {v}{where} must lie "
+                                     "in [0, 1)")
     s = table.series.get("standing_loss")
     for e in table.ids:
         if s is None or not s.covers(e):
@@ -396,7 +397,8 @@ def _reject_storage(table: Table):
                                      "concept; this model has a single horizon")
     q = _quadratic(table)
     if q:
-        raise UnsupportedNetwork(f"StorageUnit '{q[0]}' has a non-zero marginal_cost_quadratic")
+        raise UnsupportedNetwork(f"StorageUnit '{q[0]}': marginal_cost_quadratic is non-zero, "
+                                 "and a quadratic cost does not fit a linear program")
     for e in table.ids:
         mh = table.float("max_hours", e)
         if not mh > 0.0:
@@ -416,7 +418,8 @@ def _reject_stores(table: Table):
             raise UnsupportedNetwork(f"Store '{f[0]}' sets {a}, a multi-investment-period concept")
     q = _quadratic(table)
     if q:
-        raise UnsupportedNetwork(f"Store '{q[0]}' has a non-zero marginal_cost_quadratic")
+        raise UnsupportedNetwork(f"Store '{q[0]}': marginal_cost_quadratic is non-zero, and a "
+                                 "quadratic cost does not fit a linear program")
     _reject_standing_loss(table, "Store")
 
 
@@ -455,7 +458,7 @@ def build(network: Network) -> Model:
     net = expand_standard_types(network)
     unhandled = [t for t in net.tables.values() if t.size > 0 and t.name not in HANDLED]
     if unhandled:
-        raise UnsupportedNetwork("network contains unmodelled component(s): "
+        raise UnsupportedNetwork("the LP builder has no model for these components: "
                                  + ", ".join(f"{t.name} ({t.size})" for t in unhandled))
     dangling = dangling_bus_references(net)
     if dangling:
@@ -683,8 +686,8 @@ def build(network: Network) -> Model:
             sense, constant = gc.string("sense", cid), gc.float("constant", cid)
             kind = gc.string("type", cid)
             if kind != "primary_energy":
-                raise UnsupportedNetwork(f"global constraint '{cid}' has type '{kind}'; only "
-                                         "'primary_energy' is implemented")
+                raise UnsupportedNetwork(f"global constraint '{cid}': type '{kind}' is not "
+                                         "built (the builder knows 'primary_energy')")
             if sense != "<=":
                 raise UnsupportedNetwork(f"global constraint '{cid}' has sense '{sense}'; only "
                                          "'<=' is implemented")

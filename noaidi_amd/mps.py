@@ -91,12 +91,12 @@ class _Col:
 
 def _check_objsense(token: str) -> None:
     if token.upper().startswith("MAX"):
-        raise MpsParseException("OBJSENSE MAX is not supported; the solver minimises")
+        raise MpsParseException("OBJSENSE MAX: only minimisation problems are read")
 
 
 def _pairs(fields: list, line: str, section: str):
     if len(fields) % 2:
-        raise MpsParseException(f"{section} entry has an unpaired field: '{line}'")
+        raise MpsParseException(f"{section}: name/value fields do not pair up in '{line}'")
     return [(fields[i], _number(fields[i + 1], line)) for i in range(0, len(fields), 2)]
 
 
@@ -201,10 +201,10 @@ def from_lines(lines: Iterable[str], fallback_name: str = "inline") -> MpsInstan
             cname = next((t for t in cand if t in cols), None)
             if cname is None:
                 raise MpsParseException(
-                    f"BOUNDS names no column declared in COLUMNS (tried {', '.join(cand)}) "
-                    f"in: '{line}'")
+                    f"BOUNDS: none of {', '.join(cand)} is a column of the COLUMNS section "
+                    f"('{line}')")
             if len(tokens) > 4:
-                raise MpsParseException(f"BOUNDS entry carries more fields than one bound uses: "
+                raise MpsParseException(f"BOUNDS: too many fields for a single bound in "
                                         f"'{line}'")
             col = column(cname)
             value = (lambda: _number(tokens[-1], line))
@@ -238,10 +238,10 @@ def from_lines(lines: Iterable[str], fallback_name: str = "inline") -> MpsInstan
         elif section in ("ENDATA", ""):
             pass
         else:
-            raise MpsParseException(f"unexpected data in section '{section}'")
+            raise MpsParseException(f"section '{section}' does not take data lines")
 
     if not objective:
-        raise MpsParseException("no objective row (an N row) was declared")
+        raise MpsParseException("the ROWS section declares no N row to serve as objective")
     return _build(name, obj_constant, list(rows.values()), list(cols.values()))
 
 

@@ -76,7 +76,7 @@ def _kind(pypsa_type: str) -> str:
                          ("series", "float")):
         if t.startswith(prefix):
             return kind
-    raise ValueError(f"unknown PyPSA attribute type '{pypsa_type}'")
+    raise ValueError(f"PyPSA attribute type '{pypsa_type}' is not one this reader knows")
 
 
 def _default(raw, kind: str):
@@ -473,8 +473,8 @@ def branch_ports(table: Table) -> List[str]:
     if table.size > 0:
         for required in ("bus0", "bus1"):
             if required not in ports:
-                raise MalformedNetwork(f"{table.spec.list_name} has no '{required}' column; a "
-                                       "branch needs both endpoints")
+                raise MalformedNetwork(f"{table.spec.list_name}: the '{required}' column is "
+                                       "missing, and every branch joins two buses")
     return ports
 
 

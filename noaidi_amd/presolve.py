@@ -101,15 +101,15 @@ class PresolveResult:
     @property
     def reduced(self) -> prima.LpProblem:
         if self.proven_status is not None:
-            raise RuntimeError(f"presolve already established {self.proven_status}; "
-                               "there is nothing to solve")
+            raise RuntimeError(f"presolve settled the problem as {self.proven_status}; "
+                               "no reduced problem is left to solve")
         return self._reduced
 
     def restore(self, sol: prima.LpSolution) -> prima.LpSolution:
         """Map a solution of the reduced problem back (Presolve.scala:108-190)."""
         if self.proven_status is not None:
-            raise RuntimeError(f"presolve already established {self.proven_status}; restoring "
-                               "a reduced solution would fabricate one")
+            raise RuntimeError(f"presolve settled the problem as {self.proven_status}; a "
+                               "solution of a reduced problem cannot belong to it")
         p = self.original
         primal = np.zeros(p.n)
         dual = np.zeros(p.m)

@@ -116,7 +116,10 @@ class _MixedInfo(C.Structure):
 class _Stats(C.Structure):
     _fields_ = [("launches", C.c_int64), ("kernel_ms", C.c_double),
                 ("accepted_iterations", C.c_int64), ("trials", C.c_int64),
-                ("evaluations", C.c_int64), ("algorithmic_bytes", C.c_double)]
+                ("evaluations", C.c_int64), ("algorithmic_bytes", C.c_double),
+                ("launches_by_precision", C.c_int64 * 2),
+                ("kernel_ms_by_precision", C.c_double * 2),
+                ("algorithmic_bytes_by_precision", C.c_double * 2)]
 
 
 class _BnbParams(C.Structure):
@@ -798,7 +801,11 @@ class HipKernels:
     def stats(self) -> dict:
         st = _Stats()
         self._check(lib().prima_stats_get(self._h, C.byref(st)), "stats")
-        return {f[0]: getattr(st, f[0]) for f in _Stats._fields_}
+        out = {}
+        for f in _Stats._fields_:
+            v = getattr(st, f[0])
+            out[f[0]] = list(v) if f[0].endswith("_by_precision") else v
+        return out
 
     def progress(self) -> dict:
         """Launches, accepted iterations and finished LPs so far; may be called

@@ -80,13 +80,13 @@ def _reject(net: Network):
             for e in g.ids:
                 v = g.float(attribute, e)
                 if math.isfinite(v):
-                    raise UnsupportedNetwork(f"generator '{e}' has {attribute} = {v}; ramp limits "
-                                             "are not modelled")
+                    raise UnsupportedNetwork(f"generator '{e}': {attribute} = {v} asks for a "
+                                             "ramp constraint, which this model does not build")
     if "p_nom_extendable" in g.static:
         for e in g.ids:
             if g.bool("p_nom_extendable", e):
-                raise UnsupportedNetwork(f"generator '{e}' is extendable; commitment takes "
-                                         "capacity as given")
+                raise UnsupportedNetwork(f"generator '{e}': p_nom_extendable is set, but the "
+                                         "commitment model needs a fixed capacity")
 
 
 @dataclass

@@ -1,6 +1,6 @@
 """The benchmark's contract on the CPU: it refuses more than one GPU with a
-message, and every scenario of the headline batch has a HiGHS golden to be
-checked against."""
+message, the headline sweep has HiGHS goldens across its horizons, and the
+FP64 denominator samples size strata at their midpoints."""
 import os
 import subprocess
 import sys
@@ -17,9 +17,22 @@ def test_more_than_one_gpu_is_refused():
     assert r.stdout == ""
 
 
-def test_every_headline_scenario_has_a_golden():
+def test_sweep_goldens_cover_every_horizon():
+    # the headline is the uneven sweep: HiGHS goldens include instances of
+    # every snapshot count, and the relative gap is |a-b| / max(1, |b|)
     import bench
-    gold = bench.golden_objectives()
-    assert sorted(gold) == list(range(256))
+    from noaidi_amd import prima
+    gold = bench.sweep_goldens()
+    assert len(gold) >= 20
+    assert {prima.config3_args(k)["snapshots"] for k in gold} == {24, 72, 168}
     assert bench.rel_gap(1.0 + 1e-9, 1.0) <= 1e-8 < bench.rel_gap(1.0 + 2e-8, 1.0)
-    assert bench.rel_gap(1e-3, 0.0) == 1e-3  # |a-b| / max(1, |b|)
+    assert bench.rel_gap(1e-3, 0.0) == 1e-3
+
+
+def test_stratified_sample_takes_strata_midpoints():
+    import bench
+    sizes = [50, 10, 40, 20, 30, 80, 70, 60]
+    pick = bench.stratified(sizes, 4)
+    # sorted sizes 10 20 30 40 | 50 60 70 80: strata of two, midpoints the upper
+    assert [sizes[i] for i in pick] == [20, 40, 60, 80]
+    assert len(set(bench.stratified(list(range(256)), 4))) == 4

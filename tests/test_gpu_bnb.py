@@ -218,6 +218,35 @@ def test_uc_frontier_nodes_share_one_matrix(hip64):
     assert abs(s.objective_value - gold) / abs(gold) <= 1e-8, (s.objective_value, gold)
 
 
+def test_uc_frontier_batch_warm_mixed(hip64, hip32):
+    # the 100-bus UC frontier as BranchAndBound solves a frontier
+    # (BranchAndBound.scala:263-276): 32 node LPs that share K and differ in
+    # the bounds of nine fixed commitment binaries, warm-started from the
+    # parent's (the root's) primal, dual, step size and primal weight, solved
+    # in one call: float32 passes, then float64 passes warm-started from them
+    # (a warm float64 pass that has not converged after 1M iterations is
+    # raced by a cold one). Node 0 is among them. Every node Optimal and
+    # within 1e-8 of HiGHS.
+    root, g, node = mf.uc_frontier()
+    P = prima.PdhgParams(max_iterations=4_000_000)
+    r = hip64.solve(root, P)
+    assert r.status == "Optimal"
+    assert abs(r.objective_value - g["root_objective"]) / abs(g["root_objective"]) <= 1e-8
+    ks = list(range(0, 512, 16))
+    nodes = [node(k) for k in ks]
+    warm = [prima.WarmStart.of(r)] * len(ks)
+    mixed = prima.MixedParams(reduced_tolerance=1e-6, reduced_max_iterations=60_000,
+                              stall_iterations=1_000_000)
+    sols = hip32.solve_all(nodes, P, mixed=mixed, warms=warm)
+    for k, p, s in zip(ks, nodes, sols):
+        gold = g["nodes"][k]["objective"]
+        assert s.status == "Optimal", (k, s.status, s.mixed)
+        assert abs(s.objective_value - gold) / abs(gold) <= 1e-8, (k, s.objective_value, gold)
+        assert s.mixed["refined"]
+        room = 1e-12 * (1.0 + np.abs(s.primal))
+        assert np.all(s.primal >= p.l - room) and np.all(s.primal <= p.u + room)
+
+
 @pytest.mark.parametrize("name", list(mf.milp_ladder()))
 def test_milp_ladder(hip64, name):
     # the reference's MILP ladder: published objective and node count

@@ -186,6 +186,35 @@ def test_mixed_precision_f32_device(hip32):
     assert_kkt(p, s)
 
 
+def test_iterative_refinement_path(hip32):
+    # MixedParams(refinement="iterative"): float32 device passes on correction
+    # LPs (shifted bounds, shifted dual cone on inequality rows) around an
+    # FP64 outer loop, with the reference's contract kept: the answer is
+    # verified in FP64 (or comes from the warm FP64 fallback), certificates
+    # are re-checked. Optima of the reference's fixtures (several with
+    # inequality rows), config 0's closed form, a dispatch LP against HiGHS,
+    # and the two certificates.
+    ir = prima.MixedParams(refinement="iterative")
+    for name in OPTIMAL:
+        p, eobj, _, _ = fixture(name)
+        s = hip32.mixed_solve(p, P9, ir)
+        assert s.status == "Optimal", (name, s.status, s.mixed)
+        assert abs(s.objective_value - eobj) <= 1e-6 * max(1, abs(eobj)), name
+    p = prima.config0(3)
+    s = hip32.mixed_solve(p, prima.PdhgParams(), ir)
+    assert s.status == "Optimal" and gap(s.objective_value, p.known_objective) <= 1e-8
+    assert_kkt(p, s)
+    d = gold(lambda a: a.get("buses") == 20)[0]
+    p = prima.dispatch(**d["args"])
+    s = hip32.mixed_solve(p, prima.PdhgParams(max_iterations=500_000), ir)
+    assert s.status == "Optimal", (s.status, s.mixed)
+    assert gap(s.objective_value, d["objective"]) <= 1e-8
+    assert_kkt(p, s)
+    for name, status in (("infeasible", "PrimalInfeasible"), ("unbounded", "DualInfeasible")):
+        p, *_ = fixture(name)
+        assert hip32.mixed_solve(p, P9, ir).status == status
+
+
 def test_mixed_precision_fp64_device_skips_reduced_pass(hip64):
     p, *_ = fixture("economic-dispatch")
     s = hip64.mixed_solve(p, P9)
