@@ -2032,6 +2032,17 @@ int refine_batch(prima_ctx* ctx, int32_t count, const prima_lp* lps, const prima
       R.rounds++;
       R.inner_iterations += R.cres.iterations;
       R.inner_restarts += R.cres.restarts;
+      // a correction that diverged (an infeasible or unbounded LP has an
+      // infeasible or unbounded correction problem too) ends refinement:
+      // the LP takes the float64 pass from the last finite point
+      bool finite = R.cres.status == PRIMA_OPTIMAL || R.cres.status == PRIMA_ITERATION_LIMIT;
+      for (double v : R.cx) finite &= std::isfinite(v);
+      for (double v : R.cy) finite &= std::isfinite(v);
+      if (!finite) {
+        R.active = false;
+        fallback.push_back(k);
+        continue;
+      }
       for (int32_t i = 0; i < lp->n; ++i) {
         const double v = R.x[i] + R.cx[i] / R.dp;
         R.x[i] = v < lp->l[i] ? lp->l[i] : (v > lp->u[i] ? lp->u[i] : v);
