@@ -260,6 +260,20 @@ int prima_solve(prima_ctx* ctx, const prima_lp* lp, const prima_params* params,
                 const prima_warm* warm, prima_result* res, double* x_out, double* y_out,
                 double* rc_out);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (serving loops, harnesses that forbid hipMalloc). The same solve
+ * as prima_solve, for one LP, whose device memory (the LP's arena, view,
+ * launch plan and state) is carved from a 64 MiB workspace the library holds
+ * on each device it runs on, taken once per device when a context is opened
+ * there; nothing is allocated on the device. An LP that needs more than the
+ * workspace is rejected with PRIMA_EINVAL and a message giving both sizes.
+ * Every call on a device shares that device's workspace (its contents are
+ * cleared at the start of each call), so calls that overlap on one device
+ * race: one context, one thread, one call at a time. */
+int prima_solve_noalloc(prima_ctx* ctx, const prima_lp* lp, const prima_params* params,
+                        const prima_warm* warm, prima_result* res, double* x_out,
+                        double* y_out, double* rc_out);
+
 /* MixedPrecision.solve (MixedPrecision.scala:104-199). A precision-64
  * context skips the reduced pass (MixedPrecision.scala:111-113); a
  * precision-32 context runs the float32 pass on the device, re-checks a

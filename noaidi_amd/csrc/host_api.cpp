@@ -26,6 +26,8 @@ hipError_t launch_pdhg(int precision, const LpView* views, const int32_t* wg_off
                        const int32_t* wg_ent, int grid,
                        const KParams& P, hipStream_t stream);
 int threads_per_block();
+hipError_t scratch_init(int device);
+void* scratch_of(int device, size_t* bytes);
 int tile_nnz();
 int block_row_cap();
 // kernels_ops.hip
@@ -84,6 +86,7 @@ struct prima_ctx {
   double* scratch = nullptr;  // reduction partials for dot / squaredNorm
   std::atomic<int> cancel{0};
   int test_stall = 0;  // test hook: every group's barrier wait gives up
+  bool scratch_ok = false;  // this device's allocation-free workspace is known
   // progress of the running solve, readable from another thread
   std::atomic<int64_t> prog_launches{0}, prog_iterations{0}, prog_finished{0};
   int64_t prog_base = 0;  // accepted iterations of finished waves
@@ -534,6 +537,12 @@ struct LpLayout {
   size_t ylo, oylo;
 };
 
+// Caller-held device memory a wave is carved from instead of allocating.
+struct Workspace {
+  char* base = nullptr;
+  size_t bytes = 0;
+};
+
 struct Wave {
   std::vector<int> idx;  // indices into the batch
   std::vector<LpLayout> lay;
@@ -544,6 +553,7 @@ struct Wave {
   LpView* d_views = nullptr;
   int32_t* d_off = nullptr;
   int32_t* d_ent = nullptr;
+  bool owned = true;  // false when carved from a Workspace
   int grid = 0;
 };
 
@@ -741,7 +751,8 @@ std::vector<int32_t> xcd_order(size_t n, int xcds) {
 
 // Upload one wave, run it to completion in chunks, download the results.
 int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::vector<int>& idx,
-             const prima_params* p, bool certificates, int capacity) {
+             const prima_params* p, bool certificates, int capacity,
+             const Workspace* ws = nullptr) {
   if (idx.size() >= ((size_t)1 << 22))
     return fail(ctx, PRIMA_EINVAL, "more than 2^22 LPs in one wave");
   Wave W;
@@ -766,14 +777,32 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
   const size_t abort_off = st_base + idx.size() * sizeof(SolveState);
   const size_t arena_bytes = cv.off;
   auto t_up = clk::now();
+  const size_t views_bytes = (idx.size() * sizeof(LpView) + 255) & ~(size_t)255;
+  const size_t off_bytes = (((size_t)capacity + 1) * sizeof(int32_t) + 255) & ~(size_t)255;
+  const size_t ent_bytes = ((size_t)capacity + idx.size()) * sizeof(int32_t);
   void* arena = nullptr;
-  hipError_t e = hipMalloc(&arena, arena_bytes);
-  if (e != hipSuccess) return fail(ctx, PRIMA_ENOMEM, "device arena of " +
-                                             std::to_string(arena_bytes) + " bytes");
+  if (ws) {
+    // carve the wave from the caller's workspace: nothing is allocated
+    const size_t need = arena_bytes + views_bytes + off_bytes + ent_bytes;
+    if (need > ws->bytes)
+      return fail(ctx, PRIMA_EINVAL,
+                  "the LP needs " + std::to_string(need) + " bytes of device workspace; the "
+                  "allocation-free solve holds at most " + std::to_string(ws->bytes));
+    W.owned = false;
+    arena = ws->base;
+    W.d_views = (LpView*)(ws->base + arena_bytes);
+    W.d_off = (int32_t*)(ws->base + arena_bytes + views_bytes);
+    W.d_ent = (int32_t*)(ws->base + arena_bytes + views_bytes + off_bytes);
+  } else {
+    hipError_t e = hipMalloc(&arena, arena_bytes);
+    if (e != hipSuccess) return fail(ctx, PRIMA_ENOMEM, "device arena of " +
+                                               std::to_string(arena_bytes) + " bytes");
+  }
   W.arena = (char*)arena;
   struct Guard {
     Wave& w;
     ~Guard() {
+      if (!w.owned) return;
       if (w.arena) (void)hipFree(w.arena);
       if (w.d_views) (void)hipFree(w.d_views);
       if (w.d_off) (void)hipFree(w.d_off);
@@ -995,10 +1024,12 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
     v.spec_orig = h.M->spec_orig;
     W.views.push_back(v);
   }
-  HIP_TRY(ctx, hipMalloc((void**)&W.d_views, W.views.size() * sizeof(LpView)));
-  // a plan has at most `capacity` workgroups; each LP appears G times
-  HIP_TRY(ctx, hipMalloc((void**)&W.d_off, ((size_t)capacity + 1) * sizeof(int32_t)));
-  HIP_TRY(ctx, hipMalloc((void**)&W.d_ent, ((size_t)capacity + nlp) * sizeof(int32_t)));
+  if (W.owned) {
+    HIP_TRY(ctx, hipMalloc((void**)&W.d_views, W.views.size() * sizeof(LpView)));
+    // a plan has at most `capacity` workgroups; each LP appears G times
+    HIP_TRY(ctx, hipMalloc((void**)&W.d_off, ((size_t)capacity + 1) * sizeof(int32_t)));
+    HIP_TRY(ctx, hipMalloc((void**)&W.d_ent, ((size_t)capacity + nlp) * sizeof(int32_t)));
+  }
   const double upload_ms = ms_since(t_up);
 
   // chunked launches until every LP of the wave reached a status
@@ -1409,7 +1440,7 @@ int run_wave(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const std::v
 // in another wave: it restarts cold in place instead, and the twins are left
 // out.
 int run_jobs(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const prima_params* p,
-             bool certificates = true) {
+             bool certificates = true, const Workspace* ws = nullptr) {
   const int occ = ctx->occ[prec == 32 ? 0 : 1];
   const int capacity = std::max(1, ctx->cu_count * occ);
   size_t free_b = 0, total_b = 0;
@@ -1456,6 +1487,7 @@ int run_jobs(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const prima_
   };
   if (work.empty()) return PRIMA_OK;
   groups_for(work);
+  if (ws) return run_wave(ctx, prec, jobs, work, p, certificates, capacity, ws);
   if (arena_of(work) <= budget) return run_wave(ctx, prec, jobs, work, p, certificates, capacity);
   for (auto& J : jobs)
     if (J.shadow >= 0) {
@@ -2429,6 +2461,8 @@ int prima_open(int32_t device, int32_t precision, prima_ctx** out) {
     return PRIMA_ENOMEM;
   }
   ctx->coop_ok = coop && ctx->occ[0] > 0 && ctx->occ[1] > 0 && ctx->cu_count > 0;
+  // the allocation-free solve's workspace on this device (kept per device)
+  ctx->scratch_ok = scratch_init(device) == hipSuccess;
   *out = ctx;
   return PRIMA_OK;
 }
@@ -2965,6 +2999,32 @@ int prima_solve(prima_ctx* ctx, const prima_lp* lp, const prima_params* params,
   if (!lp || !res) return ctx ? fail(ctx, PRIMA_EINVAL, "null argument") : PRIMA_EINVAL;
   return prima_solve_batch(ctx, 1, lp, params, nullptr, warm, res, nullptr, &x_out, &y_out,
                            &rc_out);
+}
+
+int prima_solve_noalloc(prima_ctx* ctx, const prima_lp* lp, const prima_params* params,
+                        const prima_warm* warm, prima_result* res, double* x_out, double* y_out,
+                        double* rc_out) {
+  int rc = check_open(ctx);
+  if (rc) return rc;
+  if (!lp || !res || !params) return fail(ctx, PRIMA_EINVAL, "null argument");
+  std::string e = validate_params(params);
+  if (!e.empty()) return fail(ctx, PRIMA_EINVAL, e);
+  if (!ctx->coop_ok)
+    return fail(ctx, PRIMA_EDEVICE, "device lacks cooperative launch support for the solver");
+  size_t bytes = 0;
+  void* base = scratch_of(ctx->device, &bytes);
+  if (!ctx->scratch_ok || !base)
+    return fail(ctx, PRIMA_EDEVICE, "the allocation-free workspace of this device is unknown");
+  ctx->cancel.store(0);
+  std::vector<SolveJob> jobs(1);
+  rc = prepare_jobs(ctx, jobs, {lp}, params, {warm});
+  if (rc) return rc;
+  jobs[0].res = res;
+  jobs[0].x_out = x_out;
+  jobs[0].y_out = y_out;
+  jobs[0].rc_out = rc_out;
+  const Workspace ws{(char*)base, bytes};
+  return run_jobs(ctx, ctx->precision, jobs, params, true, &ws);
 }
 
 int prima_mixed_solve(prima_ctx* ctx, const prima_lp* lp, const prima_params* params,

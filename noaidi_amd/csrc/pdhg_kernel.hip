@@ -32,6 +32,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "pdhg_shared.h"
 #include "spmv_blocks.h"
 
@@ -977,6 +979,42 @@ hipError_t launch_pdhg(int precision, const LpView* views, const int32_t* wg_off
                                       args, 0, stream);
   return hipLaunchCooperativeKernel((const void*)pdhg_persistent<double>, dim3(grid), dim3(NT),
                                     args, 0, stream);
+}
+
+// Device workspace of the allocation-free solve (prima_solve_noalloc): one
+// LP's arena, its view, the launch plan and its state, carved from this
+// array. Each device the library runs on holds its own copy until the
+// process ends; calls that use it concurrently on one device race.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+void* scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+// Takes the address of this device's copy of `scratch` (the device must be
+// current). Called once per device, when a context is opened on it; the name
+// `scratch` itself is a host-side stand-in, never a device address.
+hipError_t scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> g(scratch_mu);
+  if (scratch_ptr[device]) return hipSuccess;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  scratch_ptr[device] = p;
+  return hipSuccess;
+}
+
+// The device address and size of this device's workspace (null before
+// scratch_init succeeded on it).
+void* scratch_of(int device, size_t* bytes) {
+  *bytes = kScratchBytes;
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> g(scratch_mu);
+  return scratch_ptr[device];
 }
 
 int threads_per_block() { return NT; }

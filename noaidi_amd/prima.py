@@ -161,6 +161,7 @@ EXPORTED = [
     "prima_stats_reset", "prima_mat_set_schedule", "prima_mat_autotune",
     "prima_default_bnb_params", "prima_milp_solve", "prima_bnb_skips_before_solving",
     "prima_bnb_safe_bound_of", "prima_progress", "prima_test_stall_barriers",
+    "prima_solve_noalloc",
 ]
 
 _lib = None
@@ -220,6 +221,8 @@ def lib():
             "prima_bnb_safe_bound_of": (d, [i32, d, d]),
             "prima_progress": (i32, [vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(i64)]),
             "prima_test_stall_barriers": (i32, [vp, i32]),
+            "prima_solve_noalloc": (i32, [vp, C.POINTER(_Lp), C.POINTER(_Params),
+                                          C.POINTER(_Warm), C.POINTER(_Result), vp, vp, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -822,6 +825,22 @@ class HipKernels:
               warm: Optional[WarmStart] = None) -> LpSolution:
         """Pdhg.solveWith(problem, params, this, warmStart) on the device."""
         return self.solve_all([problem], params, warms=[warm])[0]
+
+    def solve_noalloc(self, problem: LpProblem, params: PdhgParams = PdhgParams(),
+                      warm: Optional[WarmStart] = None) -> LpSolution:
+        """The solve of `solve`, with no device allocation (prima_solve_noalloc):
+        the LP's device memory comes from the library's per-device workspace."""
+        keep: list = []
+        lp = problem._c()
+        w = warm._c(keep) if warm is not None else None
+        res = _Result()
+        x, y, rc = np.zeros(max(problem.n, 1)), np.zeros(max(problem.m, 1)), np.zeros(max(problem.n, 1))
+        cp = params._c()
+        self._check(lib().prima_solve_noalloc(self._h, C.byref(lp), C.byref(cp),
+                                              C.byref(w) if w is not None else None,
+                                              C.byref(res), _ptr(x), _ptr(y), _ptr(rc)),
+                    "solve_noalloc")
+        return _solution(res, x[: problem.n], y[: problem.m], rc[: problem.n])
 
     def mixed_solve(self, problem: LpProblem, params: PdhgParams = PdhgParams(),
                     mixed: MixedParams = MixedParams()) -> LpSolution:
