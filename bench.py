@@ -58,10 +58,10 @@ from noaidi_amd import prima  # noqa: E402
 METRIC = "LPs/sec at 1e-8 rel. KKT: mixed-prec batched vs plain FP64 PDHG, 1 MI355X"
 PEAK_HBM_GBS = 8000.0
 GOLDEN_DIR = os.path.join(ROOT, "tests", "golden")
-# HBM bytes per algorithmic byte of pdhg_persistent, from rocprofv3
-# FETCH_SIZE and WRITE_SIZE scaled by the counters' reading of a 512 MiB
-# device copy (profiles/pdhg_persistent_config3_sweep_pmc.md)
-TRAFFIC_PER_ALGORITHMIC_BYTE = {32: None, 64: None}
+# HBM bytes per algorithmic byte of pdhg_persistent<double> on the headline
+# step, from rocprofv3 FETCH_SIZE and WRITE_SIZE scaled by the counters'
+# reading of a 512 MiB device copy (profiles/pdhg_persistent_scenarios256_pmc.md)
+TRAFFIC_PER_ALGORITHMIC_BYTE = {32: None, 64: 1.143}
 # host cores of the GPU box's share for one GPU
 CPU_CORES = 16
 
