@@ -224,7 +224,10 @@ typedef struct {
  * has workgroups to spare, and whichever concludes first (checked at launch
  * boundaries, which fall on the same iterations every run) gives the result.
  * A warm start can put the adaptive primal weight on a path that stalls for
- * hundreds of thousands of iterations where a cold start converges. */
+ * hundreds of thousands of iterations where a cold start converges.
+ * fix_threshold > 0 (iterative refinement only; no counterpart in the
+ * reference): a variable at a bound whose scaled reduced cost holds it there
+ * by more than fix_threshold is fixed in that round's correction LP. */
 typedef struct {
   double reduced_tolerance;
   int32_t reduced_max_iterations;
@@ -235,6 +238,7 @@ typedef struct {
   double inner_tolerance;
   int32_t inner_max_iterations;
   int32_t stall_iterations;
+  double fix_threshold;
 } prima_mixed_params;
 
 /* MixedPrecision.Result scalars (MixedPrecision.scala:82-91); rounds and

@@ -32,7 +32,7 @@ def build_hip() -> str:
     os.makedirs(OUT, exist_ok=True)
     objs = []
     inc = ["-I", os.path.join(ROOT, "include"), "-I", CSRC]
-    for src in ("pdhg_kernel.hip", "kernels_ops.hip"):
+    for src in ("pdhg_kernel.hip", "kernels_ops.hip", "refine_kernels.hip"):
         obj = os.path.join(OUT, src.replace(".hip", ".o"))
         _run([HIPCC, f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-c",
               *DEVICE_FLAGS, *inc, os.path.join(CSRC, src), "-o", obj])

@@ -54,6 +54,23 @@ hipError_t op_spmv_panels(int prec, int P, int nranges, const int32_t* range_row
                           void* out, hipStream_t s);
 int panel_rows();
 int panel_tile();
+// refine_kernels.hip
+int ref_partial_blocks();
+int ref_result_doubles();
+hipError_t ref_evaluate(int n, int m, int n_eq, int64_t nnz, const int32_t* aptr,
+                        const int32_t* acol, const double* aval, const int32_t* tptr,
+                        const int32_t* tcol, const double* tval, const double* x,
+                        const double* y, const double* c, const double* q, const double* l,
+                        const double* u, double eps_abs, double eps_rel, double rhs_norm,
+                        double obj_norm, double offset, double* rq, double* rc, double* part,
+                        double* out, hipStream_t s);
+hipError_t ref_build(int n, int m, int n_eq, double dp, double dd, double cap, double fix,
+                     const double* rc, const double* rq, const double* x, const double* y,
+                     const double* l, const double* u, double* cc, double* cl, double* cu,
+                     double* cq, double* ylo, hipStream_t s);
+hipError_t ref_apply(int n, int m, int n_eq, double dp, double dd, const double* cx,
+                     const double* cy, const double* l, const double* u, double* x, double* y,
+                     hipStream_t s);
 }  // namespace prima
 
 using namespace prima;
@@ -1654,6 +1671,8 @@ std::string validate_mixed(const prima_mixed_params* mx) {
                               mx->inner_max_iterations <= 0))
     return "iterative refinement needs max_refinements >= 0, inner_tolerance > 0 and "
            "inner_max_iterations > 0";
+  if (!(mx->fix_threshold >= 0.0) || std::isinf(mx->fix_threshold))
+    return "fix_threshold must be finite and >= 0";
   if (!(mx->reduced_tolerance > 0.0)) return "reducedTolerance must be positive";
   if (mx->reduced_max_iterations <= 0) return "reducedMaxIterations must be positive";
   if (!(mx->reduced_time_share > 0.0 && mx->reduced_time_share < 1.0))
@@ -1810,79 +1829,35 @@ hipError_t spmv_scheduled(prima_ctx* ctx, const prima_mat* a, const void* x, voi
 
 // ---------------------------------------------------------------------------
 // FP64 outer iterative refinement (Gleixner, Steffy & Wolter 2016) around
-// float32 device solves of correction LPs on the same matrix.
-
-struct Transposed {
-  std::vector<int32_t> ptr, col;
-  std::vector<double> val;
-};
-
-struct Kkt4 {
-  double pres, dres, pobj, dobj;
-};
-
-// Kkt.evaluate (Kkt.scala:68-125) in double precision on the original LP,
-// also returning K x and K'y.
-Kkt4 eval64(const prima_lp* lp, const Transposed& T, const double* x, const double* y,
-            std::vector<double>& ax, std::vector<double>& aty) {
-  const int32_t n = lp->n, m = lp->m;
-  ax.assign((size_t)m, 0.0);
-  aty.assign((size_t)n, 0.0);
-  for (int32_t r = 0; r < m; ++r) {
-    double acc = 0.0;
-    for (int32_t q = lp->row_ptr[r]; q < lp->row_ptr[r + 1]; ++q) acc += lp->val[q] * x[lp->col[q]];
-    ax[r] = acc;
-  }
-  for (int32_t i = 0; i < n; ++i) {
-    double acc = 0.0;
-    for (int32_t q = T.ptr[i]; q < T.ptr[i + 1]; ++q) acc += T.val[q] * y[T.col[q]];
-    aty[i] = acc;
-  }
-  double psq = 0.0;
-  for (int32_t r = 0; r < m; ++r) {
-    if (r < lp->n_eq) {
-      const double d = ax[r] - lp->q[r];
-      psq += d * d;
-    } else {
-      const double sf = lp->q[r] - ax[r];
-      if (sf > 0.0) psq += sf * sf;
-    }
-  }
-  double pobj = lp->offset, dobj = lp->offset, dsq = 0.0;
-  for (int32_t i = 0; i < n; ++i) pobj += lp->c[i] * x[i];
-  for (int32_t r = 0; r < m; ++r) dobj += lp->q[r] * y[r];
-  for (int32_t i = 0; i < n; ++i) {
-    const double rc = lp->c[i] - aty[i];
-    if (rc > 0.0) {
-      if (std::isinf(lp->l[i]) && lp->l[i] < 0) dsq += rc * rc;
-      else dobj += lp->l[i] * rc;
-    } else if (rc < 0.0) {
-      if (std::isinf(lp->u[i]) && lp->u[i] > 0) dsq += rc * rc;
-      else dobj += lp->u[i] * rc;
-    }
-  }
-  return Kkt4{std::sqrt(psq), std::sqrt(dsq), pobj, dobj};
-}
-
-bool kkt4_converged(const Kkt4& e, double rhs_norm, double obj_norm, const prima_params* p) {
-  const bool finite = std::isfinite(e.pres) && std::isfinite(e.dres) && std::isfinite(e.pobj) &&
-                      std::isfinite(e.dobj);
-  return finite && e.pres <= p->eps_abs + p->eps_rel * rhs_norm &&
-         e.dres <= p->eps_abs + p->eps_rel * obj_norm &&
-         std::fabs(e.pobj - e.dobj) <= p->eps_abs + p->eps_rel * (std::fabs(e.pobj) + std::fabs(e.dobj));
-}
+// float32 device solves of correction LPs on the same matrix. The refined
+// point (x, y), the original LP's double-precision matrix, its transpose and
+// data stay on the device; the residuals b - A x and c - A^T y, the KKT
+// error, objectives and termination test, the correction problem's data and
+// the correction step all run there in float64 (refine_kernels.hip). The
+// host reads one result record per LP and round and decides the scale of the
+// next correction.
 
 struct RefineState {
-  std::vector<double> x, y, ax, aty;
-  Transposed T;
+  std::vector<double> x, y;  // host copy of the point when it is needed there
   double dp = 1.0, dd = 1.0, rhs_norm = 0.0, obj_norm = 0.0;
   bool active = true;
   int inner_iterations = 0, inner_restarts = 0, rounds = 0;
   double step = NAN, weight = NAN;
-  // correction LP storage
-  std::vector<double> cc, cq, cl, cu, ylo, cx, cy, crc;
+  int mat = -1;  // index of the device copy of the original matrix
+  // device arrays (offsets into the refinement arena)
+  size_t dx, dy, dc, dq, dl, du, drq, drc, dcx, dcy, dcc, dcl, dcu, dcq, dylo, dpart;
+  // correction LP storage (host: the correction solve prepares it there)
+  std::vector<double> cc, cq, cl, cu, ylo, cx, cy;
   prima_lp corr{};
   prima_result cres{};
+};
+
+// Device copy of an original (unscaled) constraint matrix and its transpose.
+struct RefMat {
+  const prima_lp* lp = nullptr;
+  std::vector<int32_t> tptr, tcol;
+  std::vector<double> tval;
+  size_t aptr = 0, acol = 0, aval = 0, dtptr = 0, dtcol = 0, dtval = 0;
 };
 
 int refine_batch(prima_ctx* ctx, int32_t count, const prima_lp* lps, const prima_params* p,
@@ -1892,6 +1867,8 @@ int refine_batch(prima_ctx* ctx, int32_t count, const prima_lp* lps, const prima
   const prima_params rp = reduced_params(p, mx);
   std::vector<RefineState> S(count);
   std::vector<prima_result> red(count);
+  std::vector<uint64_t> mkey(count, 0);
+  std::vector<std::vector<double>> crc(count);
   {
     std::vector<SolveJob> jobs(count);
     {
@@ -1907,17 +1884,19 @@ int refine_batch(prima_ctx* ctx, int32_t count, const prima_lp* lps, const prima
     for (int32_t k = 0; k < count; ++k) {
       S[k].x.assign((size_t)lps[k].n, 0.0);
       S[k].y.assign((size_t)lps[k].m, 0.0);
-      S[k].crc.assign((size_t)lps[k].n, 0.0);
+      crc[k].assign((size_t)lps[k].n, 0.0);
       jobs[k].res = &red[k];
       jobs[k].x_out = S[k].x.data();
       jobs[k].y_out = S[k].y.data();
-      jobs[k].rc_out = S[k].crc.data();
+      jobs[k].rc_out = crc[k].data();
+      mkey[k] = jobs[k].h.M ? jobs[k].h.M->key : 0;
     }
     int rc = run_jobs(ctx, 32, jobs, &rp);
     if (rc) return rc;
   }
   const double reduced_ms = ms_since(t0);
   std::vector<int> fallback;
+  std::vector<RefMat> mats;
   for (int32_t k = 0; k < count; ++k) {
     prima_mixed_info& info = infos[k];
     std::memset(&info, 0, sizeof(info));
@@ -1935,7 +1914,7 @@ int refine_batch(prima_ctx* ctx, int32_t count, const prima_lp* lps, const prima
       info.refined = 0;
       if (xo && xo[k]) std::memcpy(xo[k], R.x.data(), R.x.size() * 8);
       if (yo && yo[k]) std::memcpy(yo[k], R.y.data(), R.y.size() * 8);
-      if (rco && rco[k]) std::memcpy(rco[k], R.crc.data(), R.crc.size() * 8);
+      if (rco && rco[k]) std::memcpy(rco[k], crc[k].data(), crc[k].size() * 8);
       R.active = false;
       continue;
     }
@@ -1949,15 +1928,113 @@ int refine_batch(prima_ctx* ctx, int32_t count, const prima_lp* lps, const prima
       fallback.push_back(k);
       continue;
     }
-    const prima_lp* lp = &lps[k];
-    static const int32_t zero_ptr[1] = {0};
-    transpose_csr(lp->m, lp->n, lp->m > 0 ? lp->row_ptr : zero_ptr, lp->col, lp->val, R.T.ptr,
-                  R.T.col, R.T.val);
-    R.rhs_norm = norm2(lp->q, lp->m);
-    R.obj_norm = norm2(lp->c, lp->n);
+    R.rhs_norm = norm2(lps[k].q, lps[k].m);
+    R.obj_norm = norm2(lps[k].c, lps[k].n);
     R.step = red[k].final_step_size;
     R.weight = red[k].final_primal_weight;
+    for (size_t j = 0; j < mats.size() && R.mat < 0; ++j)
+      if (mkey[mats[j].lp - lps] == mkey[k]) R.mat = (int)j;
+    if (R.mat < 0) {
+      R.mat = (int)mats.size();
+      mats.push_back(RefMat{});
+      mats.back().lp = &lps[k];
+    }
   }
+
+  // device arena: one copy of each distinct original matrix, then the
+  // float64 vectors of every refined LP, then all result records
+  static const int32_t zero_ptr[1] = {0};
+  parallel_for(mats.size(), [&](size_t j) {
+    const prima_lp* lp = mats[j].lp;
+    transpose_csr(lp->m, lp->n, lp->m > 0 ? lp->row_ptr : zero_ptr, lp->col, lp->val,
+                  mats[j].tptr, mats[j].tcol, mats[j].tval);
+  });
+  Carver cv;
+  for (RefMat& M : mats) {
+    const size_t n = (size_t)M.lp->n, m = (size_t)M.lp->m, z = (size_t)M.lp->nnz;
+    M.aptr = cv.take((m + 1) * 4);
+    M.acol = cv.take(z * 4);
+    M.aval = cv.take(z * 8);
+    M.dtptr = cv.take((n + 1) * 4);
+    M.dtcol = cv.take(z * 4);
+    M.dtval = cv.take(z * 8);
+  }
+  const size_t part_bytes = (size_t)ref_partial_blocks() * 8 * 8;
+  for (int32_t k = 0; k < count; ++k) {
+    RefineState& R = S[k];
+    if (!R.active) continue;
+    const size_t n = (size_t)lps[k].n, m = (size_t)lps[k].m;
+    R.dx = cv.take(n * 8);
+    R.dc = cv.take(n * 8);
+    R.dl = cv.take(n * 8);
+    R.du = cv.take(n * 8);
+    R.drc = cv.take(n * 8);
+    R.dcx = cv.take(n * 8);
+    R.dcc = cv.take(n * 8);
+    R.dcl = cv.take(n * 8);
+    R.dcu = cv.take(n * 8);
+    R.dy = cv.take(m * 8);
+    R.dq = cv.take(m * 8);
+    R.drq = cv.take(m * 8);
+    R.dcy = cv.take(m * 8);
+    R.dcq = cv.take(m * 8);
+    R.dylo = cv.take(m * 8);
+    R.dpart = cv.take(part_bytes);
+  }
+  const int RF = ref_result_doubles();
+  const size_t outs_off = cv.take((size_t)count * RF * 8);
+  char* arena = nullptr;
+  std::vector<double> outs((size_t)count * RF, 0.0);
+  hipStream_t s = ctx->stream;
+  auto D = [&](size_t off) { return (double*)(arena + off); };
+  auto I = [&](size_t off) { return (int32_t*)(arena + off); };
+  auto up = [&](size_t off, const void* src, size_t bytes) {
+    return bytes ? hipMemcpyAsync(arena + off, src, bytes, hipMemcpyHostToDevice, s) : hipSuccess;
+  };
+  auto down = [&](void* dst, size_t off, size_t bytes) {
+    return bytes ? hipMemcpyAsync(dst, arena + off, bytes, hipMemcpyDeviceToHost, s) : hipSuccess;
+  };
+  struct ArenaGuard {
+    char** a;
+    ~ArenaGuard() {
+      if (*a) (void)hipFree(*a);
+    }
+  } guard{&arena};
+  bool any_active = false;
+  for (int32_t k = 0; k < count; ++k) any_active |= S[k].active;
+  if (any_active) {
+    HIP_TRY(ctx, hipMalloc((void**)&arena, std::max<size_t>(cv.off, 256)));
+    for (RefMat& M : mats) {
+      const prima_lp* lp = M.lp;
+      const size_t n = (size_t)lp->n, m = (size_t)lp->m, z = (size_t)lp->nnz;
+      HIP_TRY(ctx, up(M.aptr, lp->m > 0 ? lp->row_ptr : zero_ptr, (m + 1) * 4));
+      HIP_TRY(ctx, up(M.acol, lp->col, z * 4));
+      HIP_TRY(ctx, up(M.aval, lp->val, z * 8));
+      HIP_TRY(ctx, up(M.dtptr, M.tptr.data(), (n + 1) * 4));
+      HIP_TRY(ctx, up(M.dtcol, M.tcol.data(), z * 4));
+      HIP_TRY(ctx, up(M.dtval, M.tval.data(), z * 8));
+    }
+    for (int32_t k = 0; k < count; ++k) {
+      RefineState& R = S[k];
+      if (!R.active) continue;
+      const prima_lp* lp = &lps[k];
+      const size_t n = (size_t)lp->n, m = (size_t)lp->m;
+      HIP_TRY(ctx, up(R.dx, R.x.data(), n * 8));
+      HIP_TRY(ctx, up(R.dy, R.y.data(), m * 8));
+      HIP_TRY(ctx, up(R.dc, lp->c, n * 8));
+      HIP_TRY(ctx, up(R.dl, lp->l, n * 8));
+      HIP_TRY(ctx, up(R.du, lp->u, n * 8));
+      HIP_TRY(ctx, up(R.dq, lp->q, m * 8));
+    }
+  }
+  // the device point back into the host copy (before a float64 fallback)
+  auto point_to_host = [&](int k) -> int {
+    RefineState& R = S[k];
+    HIP_TRY(ctx, down(R.x.data(), R.dx, R.x.size() * 8));
+    HIP_TRY(ctx, down(R.y.data(), R.dy, R.y.size() * 8));
+    HIP_TRY(ctx, hipStreamSynchronize(s));
+    return PRIMA_OK;
+  };
 
   prima_params ip = *p;
   ip.eps_abs = mx->inner_tolerance;
@@ -1965,30 +2042,44 @@ int refine_batch(prima_ctx* ctx, int32_t count, const prima_lp* lps, const prima
   ip.max_iterations = mx->inner_max_iterations;
   ip.time_limit_ms = -1;
   const double growth = 1e4, cost_cap = 1e6;
-  for (int round = 0; round <= mx->max_refinements; ++round) {
+  const double fix = mx->fix_threshold;
+  for (int round = 0; round <= mx->max_refinements && any_active; ++round) {
+    // float64 residuals, KKT error and termination test of every active LP
+    for (int32_t k = 0; k < count; ++k) {
+      RefineState& R = S[k];
+      if (!R.active) continue;
+      const prima_lp* lp = &lps[k];
+      const RefMat& M = mats[(size_t)R.mat];
+      HIP_TRY(ctx, ref_evaluate(lp->n, lp->m, lp->n_eq, lp->nnz, I(M.aptr), I(M.acol),
+                                D(M.aval), I(M.dtptr), I(M.dtcol), D(M.dtval), D(R.dx),
+                                D(R.dy), D(R.dc), D(R.dq), D(R.dl), D(R.du), p->eps_abs,
+                                p->eps_rel, R.rhs_norm, R.obj_norm, lp->offset, D(R.drq),
+                                D(R.drc), D(R.dpart), D(outs_off) + (size_t)k * RF, s));
+    }
+    HIP_TRY(ctx, down(outs.data(), outs_off, outs.size() * 8));
+    HIP_TRY(ctx, hipStreamSynchronize(s));
     std::vector<int> act;
     for (int32_t k = 0; k < count; ++k) {
       RefineState& R = S[k];
       if (!R.active) continue;
       const prima_lp* lp = &lps[k];
-      const Kkt4 e = eval64(lp, R.T, R.x.data(), R.y.data(), R.ax, R.aty);
-      if (kkt4_converged(e, R.rhs_norm, R.obj_norm, p)) {
+      const double* e = &outs[(size_t)k * RF];  // pres, dres, pobj, dobj, pv, dv, converged
+      if (e[6] != 0.0) {
         prima_result& r = results[k];
         std::memset(&r, 0, sizeof(r));
         r.status = PRIMA_OPTIMAL;
         r.iterations = red[k].iterations + R.inner_iterations;
         r.restarts = red[k].restarts + R.inner_restarts;
-        r.objective = e.pobj;
-        r.dual_objective = e.dobj;
-        r.primal_residual = e.pres;
-        r.dual_residual = e.dres;
+        r.objective = e[2];
+        r.dual_objective = e[3];
+        r.primal_residual = e[0];
+        r.dual_residual = e[1];
         r.final_step_size = R.step;
         r.final_primal_weight = R.weight;
         r.workgroups = red[k].workgroups;
-        if (xo && xo[k]) std::memcpy(xo[k], R.x.data(), R.x.size() * 8);
-        if (yo && yo[k]) std::memcpy(yo[k], R.y.data(), R.y.size() * 8);
-        if (rco && rco[k])
-          for (int32_t i = 0; i < lp->n; ++i) rco[k][i] = lp->c[i] - R.aty[i];
+        if (xo && xo[k]) HIP_TRY(ctx, down(xo[k], R.dx, (size_t)lp->n * 8));
+        if (yo && yo[k]) HIP_TRY(ctx, down(yo[k], R.dy, (size_t)lp->m * 8));
+        if (rco && rco[k]) HIP_TRY(ctx, down(rco[k], R.drc, (size_t)lp->n * 8));
         infos[k].rounds = R.rounds;
         infos[k].refinement_iterations = R.inner_iterations;
         R.active = false;
@@ -1997,44 +2088,27 @@ int refine_batch(prima_ctx* ctx, int32_t count, const prima_lp* lps, const prima
       if (round == mx->max_refinements) {
         R.active = false;
         fallback.push_back(k);
+        int rc = point_to_host(k);
+        if (rc) return rc;
         continue;
       }
       // scaled residual problem: primal and dual violations brought to O(1)
-      const int32_t n = lp->n, m = lp->m, ne = lp->n_eq;
-      double pv = 0.0, dv = 0.0;
-      for (int32_t r = 0; r < m; ++r) {
-        const double rr = lp->q[r] - R.ax[r];
-        pv = std::max(pv, r < ne ? std::fabs(rr) : std::max(rr, 0.0));
-      }
-      for (int32_t i = 0; i < n; ++i) {
-        const double rc = lp->c[i] - R.aty[i];
-        const double lo = lp->l[i], hi = lp->u[i];
-        const bool atl = std::isfinite(lo) && R.x[i] - lo <= 1e-12 * (1.0 + std::fabs(lo));
-        const bool atu = std::isfinite(hi) && hi - R.x[i] <= 1e-12 * (1.0 + std::fabs(hi));
-        double vi;
-        if (atl && atu) vi = 0.0;
-        else if (atl) vi = std::max(-rc, 0.0);
-        else if (atu) vi = std::max(rc, 0.0);
-        else vi = std::fabs(rc);
-        dv = std::max(dv, vi);
-      }
-      R.dp = std::min(1.0 / std::max(pv, 1e-300), growth * R.dp);
-      R.dd = std::min(1.0 / std::max(dv, 1e-300), growth * R.dd);
+      R.dp = std::min(1.0 / std::max(e[4], 1e-300), growth * R.dp);
+      R.dd = std::min(1.0 / std::max(e[5], 1e-300), growth * R.dd);
+      const int32_t n = lp->n, m = lp->m;
+      HIP_TRY(ctx, ref_build(n, m, lp->n_eq, R.dp, R.dd, cost_cap, fix, D(R.drc), D(R.drq), D(R.dx),
+                             D(R.dy), D(R.dl), D(R.du), D(R.dcc), D(R.dcl), D(R.dcu), D(R.dcq),
+                             D(R.dylo), s));
       R.cc.resize(n);
       R.cl.resize(n);
       R.cu.resize(n);
       R.cq.resize(m);
-      R.ylo.assign((size_t)m, 0.0);
-      for (int32_t i = 0; i < n; ++i) {
-        const double c = R.dd * (lp->c[i] - R.aty[i]);
-        R.cc[i] = std::max(-cost_cap, std::min(cost_cap, c));
-        R.cl[i] = R.dp * (lp->l[i] - R.x[i]);
-        R.cu[i] = R.dp * (lp->u[i] - R.x[i]);
-        if (R.cl[i] > 0.0) R.cl[i] = 0.0;  // x is inside its box
-        if (R.cu[i] < 0.0) R.cu[i] = 0.0;
-      }
-      for (int32_t r = 0; r < m; ++r) R.cq[r] = R.dp * (lp->q[r] - R.ax[r]);
-      for (int32_t r = ne; r < m; ++r) R.ylo[r] = -R.dd * R.y[r];
+      R.ylo.resize(m);
+      HIP_TRY(ctx, down(R.cc.data(), R.dcc, (size_t)n * 8));
+      HIP_TRY(ctx, down(R.cl.data(), R.dcl, (size_t)n * 8));
+      HIP_TRY(ctx, down(R.cu.data(), R.dcu, (size_t)n * 8));
+      HIP_TRY(ctx, down(R.cq.data(), R.dcq, (size_t)m * 8));
+      HIP_TRY(ctx, down(R.ylo.data(), R.dylo, (size_t)m * 8));
       R.corr = *lp;
       R.corr.c = R.cc.data();
       R.corr.q = R.cq.data();
@@ -2045,12 +2119,18 @@ int refine_batch(prima_ctx* ctx, int32_t count, const prima_lp* lps, const prima
       R.cy.assign((size_t)m, 0.0);
       act.push_back(k);
     }
+    HIP_TRY(ctx, hipStreamSynchronize(s));
     if (act.empty()) break;
     std::vector<SolveJob> jobs(act.size());
+    {
+      std::vector<const prima_lp*> lp_of(act.size());
+      for (size_t j = 0; j < act.size(); ++j) lp_of[j] = &S[act[j]].corr;
+      int rc = prepare_jobs(ctx, jobs, lp_of, &ip,
+                            std::vector<const prima_warm*>(act.size(), nullptr));
+      if (rc) return rc;
+    }
     for (size_t j = 0; j < act.size(); ++j) {
       RefineState& R = S[act[j]];
-      int rc = prepare_job(ctx, jobs[j], &R.corr, &ip, nullptr);
-      if (rc) return rc;
       jobs[j].ylo = R.ylo.data();
       jobs[j].res = &R.cres;
       jobs[j].x_out = R.cx.data();
@@ -2073,18 +2153,17 @@ int refine_batch(prima_ctx* ctx, int32_t count, const prima_lp* lps, const prima
       if (!finite) {
         R.active = false;
         fallback.push_back(k);
+        int rc2 = point_to_host(k);
+        if (rc2) return rc2;
         continue;
       }
-      for (int32_t i = 0; i < lp->n; ++i) {
-        const double v = R.x[i] + R.cx[i] / R.dp;
-        R.x[i] = v < lp->l[i] ? lp->l[i] : (v > lp->u[i] ? lp->u[i] : v);
-      }
-      for (int32_t r = 0; r < lp->m; ++r) {
-        const double v = R.y[r] + R.cy[r] / R.dd;
-        R.y[r] = (r >= lp->n_eq && v < 0.0) ? 0.0 : v;
-      }
+      HIP_TRY(ctx, up(R.dcx, R.cx.data(), (size_t)lp->n * 8));
+      HIP_TRY(ctx, up(R.dcy, R.cy.data(), (size_t)lp->m * 8));
+      HIP_TRY(ctx, ref_apply(lp->n, lp->m, lp->n_eq, R.dp, R.dd, D(R.dcx), D(R.dcy), D(R.dl),
+                             D(R.du), D(R.dx), D(R.dy), s));
     }
   }
+  HIP_TRY(ctx, hipStreamSynchronize(s));
   const double refine_ms = ms_since(t0) - reduced_ms;
   for (int32_t k = 0; k < count; ++k) {
     if (std::find(fallback.begin(), fallback.end(), k) != fallback.end()) continue;
@@ -2099,12 +2178,21 @@ int refine_batch(prima_ctx* ctx, int32_t count, const prima_lp* lps, const prima
   std::vector<prima_warm> fw(fallback.size());
   std::vector<SolveJob> jobs(fallback.size());
   std::vector<prima_result> fres(fallback.size());
+  {
+    std::vector<const prima_lp*> lp_of(fallback.size());
+    std::vector<const prima_warm*> warm_of(fallback.size());
+    for (size_t j = 0; j < fallback.size(); ++j) {
+      const int k = fallback[j];
+      RefineState& R = S[k];
+      fw[j] = prima_warm{R.x.data(), R.y.data(), R.step, R.weight};
+      lp_of[j] = &lps[k];
+      warm_of[j] = &fw[j];
+    }
+    int rc = prepare_jobs(ctx, jobs, lp_of, p, warm_of);
+    if (rc) return rc;
+  }
   for (size_t j = 0; j < fallback.size(); ++j) {
     const int k = fallback[j];
-    RefineState& R = S[k];
-    fw[j] = prima_warm{R.x.data(), R.y.data(), R.step, R.weight};
-    int rc = prepare_job(ctx, jobs[j], &lps[k], p, &fw[j]);
-    if (rc) return rc;
     jobs[j].res = &fres[j];
     jobs[j].x_out = xo ? xo[k] : nullptr;
     jobs[j].y_out = yo ? yo[k] : nullptr;

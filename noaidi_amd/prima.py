@@ -103,7 +103,7 @@ class _Mixed(C.Structure):
                 ("always_refine", C.c_int32), ("reduced_time_share", C.c_double),
                 ("refinement", C.c_int32), ("max_refinements", C.c_int32),
                 ("inner_tolerance", C.c_double), ("inner_max_iterations", C.c_int32),
-                ("stall_iterations", C.c_int32)]
+                ("stall_iterations", C.c_int32), ("fix_threshold", C.c_double)]
 
 
 class _MixedInfo(C.Structure):
@@ -508,6 +508,7 @@ class MixedParams:
     inner_tolerance: float = 1e-4
     inner_max_iterations: int = 20_000
     stall_iterations: int = 0  # 0: the reference contract (no cold fallback)
+    fix_threshold: float = 0.0  # iterative refinement: fix strongly active bounds (0: off)
 
     def _c(self) -> _Mixed:
         m = _Mixed()
@@ -522,6 +523,7 @@ class MixedParams:
         m.inner_tolerance = self.inner_tolerance
         m.inner_max_iterations = self.inner_max_iterations
         m.stall_iterations = self.stall_iterations
+        m.fix_threshold = self.fix_threshold
         return m
 
 

@@ -338,6 +338,29 @@ def test_uneven_sweep_batch_against_highs(hip32):
         assert_kkt(p, s)
 
 
+@pytest.mark.timeout(900)
+def test_uneven_sweep_horizons_against_highs(hip32):
+    # the size-strata midpoints of the sweep with HiGHS goldens beyond the
+    # smallest: 23k to 359k variables, 24, 72 and 168 snapshots, in one
+    # mixed-precision batch; every LP Optimal, within 1e-8 of HiGHS, and its
+    # returned point meets the KKT test on the original data
+    with open(os.path.join(os.path.dirname(__file__), "golden", "highs_sweep.json")) as f:
+        gold = [d for d in json.load(f)[20:] if d["status"] == 0]
+    probs = [prima.config3(d["args"]["k"]) for d in gold]
+    snaps = {d["config3_args"]["snapshots"] for d in gold}
+    assert snaps == {24, 72, 168}, snaps
+    for p, d in zip(probs, gold):
+        assert (p.n, p.m, p.nnz) == (d["n"], d["m"], d["nnz"])
+    sols = hip32.solve_all(probs, prima.PdhgParams(max_iterations=4_000_000),
+                           mixed=prima.MixedParams(reduced_tolerance=1e-6,
+                                                   reduced_max_iterations=60_000,
+                                                   stall_iterations=1_000_000))
+    for p, d, s in zip(probs, gold, sols):
+        assert s.status == "Optimal", (d["args"], s.status, s.mixed)
+        assert gap(s.objective_value, d["objective"]) <= 1e-8, (d["args"], s.objective_value)
+        assert_kkt(p, s)
+
+
 def test_batch_beyond_capacity_admits_waiting_lps(hip64):
     # more LPs than co-resident workgroups: the extra LPs wait and take over
     # the workgroups of LPs that finish. With one workgroup per LP every LP's
