@@ -264,6 +264,33 @@ int prima_solve(prima_ctx* ctx, const prima_lp* lp, const prima_params* params,
                 const prima_warm* warm, prima_result* res, double* x_out, double* y_out,
                 double* rc_out);
 
+/* No counterpart in the reference (its only GPU SpMV, SpmvSpike.scala:43-75,
+ * has a static maxRowNnz cap): an autotuner for the solver's launch
+ * thresholds on one matrix. It runs `iterations` accepted iterations of the
+ * LP (no tolerance met, no certificate tested) at each candidate: group sizes
+ * of half, once and twice the rule of about 1k nonzeros per workgroup, each
+ * with the coarse row-block cut (TILE nonzeros, ROWCAP rows per block) and
+ * with about one and two row blocks per workgroup of the group. The device
+ * time per accepted iteration of each candidate (the kernel's HIP events)
+ * goes into the result; the fastest is kept on this context for the
+ * matrix (by content) and used by later solves: the group size of the LP
+ * solved alone (and the most it is regrouped to in a batch), and the block
+ * cut of its groups. Iteration counts to a tolerance can change with these
+ * settings (the order of the global sums changes); results stay
+ * reproducible for fixed settings. */
+#define PRIMA_TUNE_MAX_CANDIDATES 9
+typedef struct {
+  int32_t workgroups;            /* chosen group size */
+  int32_t blocks_per_workgroup;  /* chosen cut: 0 coarse, k about k blocks per workgroup */
+  double us_per_iteration;       /* its device time per accepted iteration */
+  int32_t candidates;            /* candidates timed (entries filled below) */
+  int32_t cand_workgroups[PRIMA_TUNE_MAX_CANDIDATES];
+  int32_t cand_blocks_per_workgroup[PRIMA_TUNE_MAX_CANDIDATES];
+  double cand_us_per_iteration[PRIMA_TUNE_MAX_CANDIDATES];
+} prima_tune_result;
+int prima_tune_solver(prima_ctx* ctx, const prima_lp* lp, const prima_params* params,
+                      int32_t iterations, prima_tune_result* out);
+
 /* No counterpart in the reference: for callers that cannot allocate device
  * memory (serving loops, harnesses that forbid hipMalloc). The same solve
  * as prima_solve, for one LP, whose device memory (the LP's arena, view,

@@ -88,6 +88,12 @@ struct MatrixPrep {
   double spec_scaled = 0, spec_orig = 0;
   std::vector<int32_t> a_rb, t_rb;    // block start rows (blocks + 1)
   std::vector<int32_t> a_rbd, t_rbd;  // device descriptors
+  // choices of prima_tune_solver for this matrix (copied from the context's
+  // table when the matrix is prepared): group size of an LP solved alone
+  // (0: the nonzeros rule) and row blocks per workgroup of a group (0: the
+  // coarse cut, k: about k blocks per workgroup; -1: the default, 1)
+  mutable int tuned_G = 0;
+  mutable int tuned_bpw = -1;
 };
 }  // namespace prima
 
@@ -113,6 +119,13 @@ struct prima_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // recently prepared matrices, by content (LPs that share K reuse them)
   std::vector<std::shared_ptr<const prima::MatrixPrep>> prep_cache;
+  // solver settings chosen by prima_tune_solver, by matrix content key:
+  // {key, group size, row blocks per workgroup}
+  struct Tuned {
+    uint64_t key;
+    int G, bpw;
+  };
+  std::vector<Tuned> tuned;
 };
 
 struct prima_vec {
@@ -403,13 +416,15 @@ std::vector<int32_t> partition(const std::vector<int32_t>& rb, const int32_t* pt
 // rows or 512 nonzeros); a single workgroup uses the shared coarse cut.
 void group_blocks(const MatrixPrep& M, const int32_t* aptr, int32_t n, int32_t m, int64_t nnz,
                   int G, std::vector<int32_t>& a_rb, std::vector<int32_t>& t_rb) {
-  if (G <= 1) {
+  const int bpw = M.tuned_bpw < 0 ? 1 : M.tuned_bpw;
+  if (G <= 1 || bpw == 0) {
     a_rb = M.a_rb;
     t_rb = M.t_rb;
     return;
   }
+  const int64_t parts = (int64_t)G * bpw;
   auto cap = [&](int64_t total, int64_t lo, int64_t hi) {
-    return (int)std::max<int64_t>(lo, std::min<int64_t>(hi, (total + G - 1) / G));
+    return (int)std::max<int64_t>(lo, std::min<int64_t>(hi, (total + parts - 1) / parts));
   };
   const int tile = cap(nnz, 512, tile_nnz());
   a_rb = row_blocks(m, aptr, tile, cap(m, 64, block_row_cap()));
@@ -688,7 +703,8 @@ void choose_groups(const prima_params* p, const std::vector<SolveJob*>& js, int 
   const int gmax = std::max(1, std::min(capacity, kMaxGroup));
   for (SolveJob* J : js) {
     const int64_t nnz = std::max<int64_t>(J->h.nnz, 1);
-    const int alone = (int)std::max<int64_t>(1, std::min<int64_t>((nnz + 1023) / 1024, gmax));
+    const int64_t rule = J->h.M && J->h.M->tuned_G > 0 ? J->h.M->tuned_G : (nnz + 1023) / 1024;
+    const int alone = (int)std::max<int64_t>(1, std::min<int64_t>(rule, gmax));
     J->h.G_cap = std::max(J->h.G_cap, alone);
     if (p->workgroups_per_lp > 0) {
       J->h.G = std::min(p->workgroups_per_lp, gmax);
@@ -1620,6 +1636,16 @@ int prepare_jobs(prima_ctx* ctx, std::vector<SolveJob>& jobs,
     }
     std::reverse(keep.begin(), keep.end());
     ctx->prep_cache.swap(keep);
+  }
+  // settings prima_tune_solver chose for these matrices
+  for (size_t j = 0; j < N; ++j) {
+    prep[j]->tuned_G = 0;
+    prep[j]->tuned_bpw = -1;
+    for (const auto& t : ctx->tuned)
+      if (t.key == prep[j]->key) {
+        prep[j]->tuned_G = t.G;
+        prep[j]->tuned_bpw = t.bpw;
+      }
   }
   parallel_for(N, [&](size_t j) {
     SolveJob& J = jobs[j];
@@ -2710,6 +2736,13 @@ static int build_panels(prima_ctx* ctx, prima_mat* M, const int32_t* row_ptr, co
   if ((int64_t)M->cols * (prec == 32 ? 4 : 8) < 4 * (1 << 20)) return PRIMA_OK;
   const int P = (int)((M->cols + W - 1) / W);
   const int R = panel_rows(), TP = panel_tile();
+  // The layout pays for scattered columns; it costs 2 bytes per row per
+  // panel of row starts and P steps per row to build. A matrix whose row
+  // starts would outweigh its column indices (many panels, few entries per
+  // row, as in a block-diagonal batch of LPs whose gathers are already
+  // local) keeps CSR only.
+  if (P > 256 || (double)P * ((double)M->rows / R + 1) * (R + 1) * 2 > 4.0 * (double)M->nnz)
+    return PRIMA_OK;
   const int LONG = 512;
   const int rows = M->rows;
   std::vector<int32_t> range_row0{0}, long_rows;
@@ -3087,6 +3120,69 @@ int prima_solve(prima_ctx* ctx, const prima_lp* lp, const prima_params* params,
   if (!lp || !res) return ctx ? fail(ctx, PRIMA_EINVAL, "null argument") : PRIMA_EINVAL;
   return prima_solve_batch(ctx, 1, lp, params, nullptr, warm, res, nullptr, &x_out, &y_out,
                            &rc_out);
+}
+
+int prima_tune_solver(prima_ctx* ctx, const prima_lp* lp, const prima_params* params,
+                      int32_t iterations, prima_tune_result* out) {
+  int rc = check_open(ctx);
+  if (rc) return rc;
+  if (!lp || !params || !out || iterations <= 0)
+    return fail(ctx, PRIMA_EINVAL, "bad tuning arguments");
+  std::string e = validate_lp(lp);
+  if (e.empty()) e = validate_params(params);
+  if (!e.empty()) return fail(ctx, PRIMA_EINVAL, e);
+  std::memset(out, 0, sizeof(*out));
+  const uint64_t key = matrix_key(lp, params);
+  // forget an earlier choice for this matrix while the candidates run
+  ctx->tuned.erase(std::remove_if(ctx->tuned.begin(), ctx->tuned.end(),
+                                  [&](const prima_ctx::Tuned& t) { return t.key == key; }),
+                   ctx->tuned.end());
+  const int occ = ctx->occ[ctx->precision == 32 ? 0 : 1];
+  const int gmax = std::max(1, std::min(ctx->cu_count * std::max(occ, 1), kMaxGroup));
+  const int64_t nnz = std::max<int64_t>(lp->nnz, 1);
+  const int rule = (int)std::max<int64_t>(1, std::min<int64_t>((nnz + 1023) / 1024, gmax));
+  std::vector<int> gs;
+  for (int g : {std::max(1, rule / 2), rule, std::min(gmax, 2 * rule)})
+    if (std::find(gs.begin(), gs.end(), g) == gs.end()) gs.push_back(g);
+  // every candidate runs exactly `iterations` accepted iterations: no
+  // tolerance is ever met, certificates are not tested
+  prima_params tp = *params;
+  tp.eps_abs = 0.0;
+  tp.eps_rel = 0.0;
+  tp.infeasibility_tolerance = 0.0;
+  tp.max_iterations = iterations;
+  tp.time_limit_ms = -1;
+  int best = -1;
+  double best_us = INFINITY;
+  for (int g : gs) {
+    for (int bpw : {0, 1, 2}) {
+      if (g == 1 && bpw != 0) continue;  // one workgroup always takes the coarse cut
+      if (out->candidates >= PRIMA_TUNE_MAX_CANDIDATES) break;
+      ctx->tuned.push_back(prima_ctx::Tuned{key, g, bpw});
+      tp.workgroups_per_lp = g;
+      const prima_stats before = ctx->stats;
+      prima_result r{};
+      rc = prima_solve(ctx, lp, &tp, nullptr, &r, nullptr, nullptr, nullptr);
+      ctx->tuned.pop_back();
+      if (rc) return rc;
+      const double ms = ctx->stats.kernel_ms - before.kernel_ms;
+      const double us = r.iterations > 0 ? 1e3 * ms / r.iterations : INFINITY;
+      const int c = out->candidates++;
+      out->cand_workgroups[c] = g;
+      out->cand_blocks_per_workgroup[c] = bpw;
+      out->cand_us_per_iteration[c] = us;
+      if (us < best_us) {
+        best_us = us;
+        best = c;
+      }
+    }
+  }
+  if (best < 0) return fail(ctx, PRIMA_EDEVICE, "no tuning candidate ran");
+  out->workgroups = out->cand_workgroups[best];
+  out->blocks_per_workgroup = out->cand_blocks_per_workgroup[best];
+  out->us_per_iteration = best_us;
+  ctx->tuned.push_back(prima_ctx::Tuned{key, out->workgroups, out->blocks_per_workgroup});
+  return PRIMA_OK;
 }
 
 int prima_solve_noalloc(prima_ctx* ctx, const prima_lp* lp, const prima_params* params,

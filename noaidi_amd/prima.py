@@ -113,6 +113,17 @@ class _MixedInfo(C.Structure):
                 ("rounds", C.c_int32), ("fallback_iterations", C.c_int32)]
 
 
+TUNE_MAX = 9
+
+
+class _TuneResult(C.Structure):
+    _fields_ = [("workgroups", C.c_int32), ("blocks_per_workgroup", C.c_int32),
+                ("us_per_iteration", C.c_double), ("candidates", C.c_int32),
+                ("cand_workgroups", C.c_int32 * TUNE_MAX),
+                ("cand_blocks_per_workgroup", C.c_int32 * TUNE_MAX),
+                ("cand_us_per_iteration", C.c_double * TUNE_MAX)]
+
+
 class _Stats(C.Structure):
     _fields_ = [("launches", C.c_int64), ("kernel_ms", C.c_double),
                 ("accepted_iterations", C.c_int64), ("trials", C.c_int64),
@@ -161,7 +172,7 @@ EXPORTED = [
     "prima_stats_reset", "prima_mat_set_schedule", "prima_mat_autotune",
     "prima_default_bnb_params", "prima_milp_solve", "prima_bnb_skips_before_solving",
     "prima_bnb_safe_bound_of", "prima_progress", "prima_test_stall_barriers",
-    "prima_solve_noalloc",
+    "prima_solve_noalloc", "prima_tune_solver",
 ]
 
 _lib = None
@@ -223,6 +234,8 @@ def lib():
             "prima_test_stall_barriers": (i32, [vp, i32]),
             "prima_solve_noalloc": (i32, [vp, C.POINTER(_Lp), C.POINTER(_Params),
                                           C.POINTER(_Warm), C.POINTER(_Result), vp, vp, vp]),
+            "prima_tune_solver": (i32, [vp, C.POINTER(_Lp), C.POINTER(_Params), i32,
+                                        C.POINTER(_TuneResult)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -843,6 +856,23 @@ class HipKernels:
                                               C.byref(res), _ptr(x), _ptr(y), _ptr(rc)),
                     "solve_noalloc")
         return _solution(res, x[: problem.n], y[: problem.m], rc[: problem.n])
+
+    def tune_solver(self, problem: LpProblem, params: PdhgParams = PdhgParams(),
+                    iterations: int = 1024) -> dict:
+        """prima_tune_solver: time `iterations` iterations of the LP at each
+        candidate group size and row-block cut, keep the fastest for later
+        solves of this matrix on this context, and return every timing."""
+        lp = problem._c()
+        cp = params._c()
+        out = _TuneResult()
+        self._check(lib().prima_tune_solver(self._h, C.byref(lp), C.byref(cp), int(iterations),
+                                            C.byref(out)), "tune_solver")
+        n = out.candidates
+        return {"workgroups": out.workgroups,
+                "blocks_per_workgroup": out.blocks_per_workgroup,
+                "us_per_iteration": out.us_per_iteration,
+                "candidates": [(out.cand_workgroups[i], out.cand_blocks_per_workgroup[i],
+                                out.cand_us_per_iteration[i]) for i in range(n)]}
 
     def mixed_solve(self, problem: LpProblem, params: PdhgParams = PdhgParams(),
                     mixed: MixedParams = MixedParams()) -> LpSolution:

@@ -90,7 +90,8 @@ def test_struct_layouts_match_header(tmp_path):
                "prima_result": prima._Result, "prima_warm": prima._Warm,
                "prima_mixed_params": prima._Mixed, "prima_mixed_info": prima._MixedInfo,
                "prima_stats": prima._Stats, "prima_capabilities": prima._Caps,
-               "prima_bnb_params": prima._BnbParams, "prima_milp_result": prima._MilpResult}
+               "prima_bnb_params": prima._BnbParams, "prima_milp_result": prima._MilpResult,
+               "prima_tune_result": prima._TuneResult}
     lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "prima_hip.h"',
              "int main(void) {"]
     for cname, py in structs.items():
@@ -117,6 +118,7 @@ def test_null_context_is_rejected_without_a_device():
     assert L.prima_close(None) == prima.PRIMA_EINVAL
     assert L.prima_cancel(None) == prima.PRIMA_EINVAL
     assert L.prima_vec_length(None) == -1
+    assert L.prima_tune_solver(None, None, None, 1, None) == prima.PRIMA_EINVAL
 
 
 def test_builder_validation():

@@ -361,6 +361,33 @@ def test_uneven_sweep_horizons_against_highs(hip32):
         assert_kkt(p, s)
 
 
+def test_solver_autotuner():
+    # prima_tune_solver on the 200-bus dispatch LP (65k nonzeros, so group
+    # sizes 32, 64 and 128, each with three row-block cuts): every candidate
+    # timed, the fastest kept on the context; the next solve of that matrix
+    # runs alone on the chosen group size and is Optimal within 1e-8 of HiGHS
+    # with the KKT check. A context of its own keeps the choice out of the
+    # other tests.
+    d = gold(lambda a: a == {"seed": 1})[0]
+    p = prima.dispatch(**d["args"])
+    with prima.HipKernels(0, 64) as k:
+        t = k.tune_solver(p, prima.PdhgParams(), iterations=512)
+        us = [c[2] for c in t["candidates"]]
+        assert len(us) == 9 and all(np.isfinite(us)) and min(us) == t["us_per_iteration"]
+        assert {c[0] for c in t["candidates"]} == {32, 64, 128}
+        g, b, _ = min(t["candidates"], key=lambda c: c[2])
+        assert (t["workgroups"], t["blocks_per_workgroup"]) == (g, b)
+        s = k.solve(p, prima.PdhgParams(max_iterations=500_000))
+        assert s.workgroups == t["workgroups"]
+        assert s.status == "Optimal", s.status
+        assert gap(s.objective_value, d["objective"]) <= 1e-8
+        assert_kkt(p, s)
+        again = k.solve(p, prima.PdhgParams(max_iterations=500_000))
+        assert again.iterations == s.iterations and again.objective_value == s.objective_value
+        with pytest.raises(ValueError):
+            k.tune_solver(p, prima.PdhgParams(), iterations=0)
+
+
 def test_batch_beyond_capacity_admits_waiting_lps(hip64):
     # more LPs than co-resident workgroups: the extra LPs wait and take over
     # the workgroups of LPs that finish. With one workgroup per LP every LP's
