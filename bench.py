@@ -539,6 +539,45 @@ SCHEDULES = {0: "auto", 1: "vector-1", 2: "vector-2", 3: "vector-4", 4: "vector-
              5: "vector-16", 6: "vector-32", 7: "vector-64", 8: "row-blocks", 9: "panels"}
 
 
+def config2_passes(args):
+    """The large random LP (2M variables, 1M rows, 20M nonzeros, lognormal
+    row lengths capped at 2048, known optimum c'x* = b'y*) in explicit
+    mixed-precision passes, each under a time limit: a float32 pass to 1e-6,
+    a float64 pass to 1e-6 warm-started from it, a float64 pass to 1e-8
+    warm-started from that. Each pass reports its status, iterations, time
+    and the relative error of its objective against the known optimum."""
+    t = time.perf_counter()
+    p = prima.config2()
+    log(f"config 2: n={p.n} m={p.m} nnz={p.nnz} known={p.known_objective!r} "
+        f"(built in {time.perf_counter() - t:.1f} s)")
+    lim = args.pass_seconds
+
+    def rep(what, s, dt):
+        return {"pass": what, "status": s.status, "iterations": s.iterations, "seconds": dt,
+                "ms_per_iteration": 1e3 * dt / max(1, s.iterations),
+                "objective": s.objective_value, "dual_objective": s.dual_objective_value,
+                "rel_err_vs_known": rel_gap(s.objective_value, p.known_objective),
+                "primal_residual": s.primal_residual, "dual_residual": s.dual_residual,
+                "workgroups": s.workgroups}
+
+    out = {"workload": f"config 2: random LP {p.n} vars, {p.m} rows, {p.nnz} nnz",
+           "known_objective": p.known_objective, "passes": []}
+    with prima.HipKernels(0, 32) as k32, prima.HipKernels(0, 64) as k64:
+        plan = [(k32, "float32 to 1e-6", 1e-6, lim[0]), (k64, "float64 to 1e-6, warm", 1e-6, lim[1]),
+                (k64, "float64 to 1e-8, warm", 1e-8, lim[2])]
+        warm = None
+        for k, what, eps, sec in plan:
+            P = prima.PdhgParams(eps_abs=eps, eps_rel=eps, max_iterations=4_000_000,
+                                 time_limit_ms=int(sec * 1e3))
+            t = time.perf_counter()
+            s = run_watched(k, lambda: k.solve(p, P, warm=warm), what)
+            r = rep(what, s, time.perf_counter() - t)
+            log(json.dumps(r))
+            out["passes"].append(r)
+            warm = prima.WarmStart.of(s)
+    print(json.dumps(out), flush=True)
+
+
 def spmv_roofline(args, reps: int = 20):
     """The per-op SpMV at the metric's scales, every schedule timed by the
     autotuner (HIP events), algorithmic bytes per SpMV Z(s+4) + 4(R+1) +
@@ -619,15 +658,19 @@ def main():
     ap.add_argument("--stall-iterations", type=int, default=1_000_000)
     ap.add_argument("--iterations", type=int, default=2048,
                     help="iterations per LP for --workload kernel")
+    ap.add_argument("--pass-seconds", type=float, nargs=3, default=[240.0, 186.7, 373.3],
+                    help="time limits of the three passes of --workload config2")
     ap.add_argument("--workload", default="scenarios",
-                    choices=["sweep", "scenarios", "uc-frontier", "kernel", "lopf", "spmv"])
+                    choices=["sweep", "scenarios", "uc-frontier", "kernel", "lopf", "spmv",
+                             "config2"])
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on exactly one GPU: this build solves independent LPs per GPU "
               "and has no multi-GPU path to measure", file=sys.stderr)
         sys.exit(2)
     {"sweep": sweep, "scenarios": scenarios, "uc-frontier": uc_frontier,
-     "kernel": kernel_profile, "lopf": lopf_scigrid, "spmv": spmv_roofline}[args.workload](args)
+     "kernel": kernel_profile, "lopf": lopf_scigrid, "spmv": spmv_roofline,
+     "config2": config2_passes}[args.workload](args)
 
 
 if __name__ == "__main__":
