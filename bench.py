@@ -3,18 +3,19 @@
 Headline workload (the default, --workload scenarios): 256 demand scenarios
 of the 200-bus, 24-snapshot PyPSA-shaped dispatch LP (the config-1 LP: 34,800
 variables, 14,400 rows, 64,750 nonzeros; per-bus demand x U[0.8, 1.2] with
-seed 1000 + k). One step solves all 256 in one batched call with mixed
-precision: a float32 device pass, then a float64 device pass warm-started
-from it (the reference's MixedPrecision contract), every LP in the same
+seed 1000 + k). The metric is a batched one and config 1 is its single-GPU
+configuration; the uneven sweep (config 3) takes more than 19 minutes to
+1e-8 and does not fit a 600 s run. One step solves all 256 in one batched
+call with mixed precision: a float32 device pass, then a float64 device pass
+warm-started from it (the reference's MixedPrecision contract), every LP in the same
 persistent launches with its own step size, restarts and termination. Every
 LP must come back Optimal and within 1e-8 (|a-b|/max(1,|b|)) of its HiGHS
 objective (tests/golden/highs_dispatch.json), or the run fails.
 
 --workload sweep runs the same measurement on the uneven scenario sweep
 (prima.config3, k = 0..--batch-1: buses log-uniform in [50, 2000], snapshots
-in {24, 72, 168}; 88.1M variables over 256 LPs). To 1e-8 its largest LPs need
-far more than a million float64 iterations each, which is longer than the
-benchmark's time limit.
+in {24, 72, 168}; 88.1M variables over 256 LPs). Its largest LPs (up to 2.4M
+variables) keep the batch running for more than 19 minutes.
 
 Timing follows the driver's fixed command (--steps K --warmup W): warm-up is
 capped at one small solve per context (the persistent kernel has no JIT to
@@ -27,14 +28,16 @@ metric's denominator) on a sample stratified by the FP64 iteration counts
 CPU path) on the host cores. One JSON line, printed and flushed at the end.
 
   python bench.py [--steps K] [--warmup W] [--budget S] [--no-cpu]
-                  [--workload scenarios|sweep|uc-frontier|kernel|lopf|spmv]
+                  [--workload scenarios|sweep|uc-frontier|kernel|lopf|spmv|config2]
 
 --workload uc-frontier: the 100-bus UC relaxation's depth-9 branch-and-bound
 frontier (--batch nodes, shared K, warm-started from the root). --workload
 kernel: the solver kernel alone for profiling (dispatch LPs for --iterations
 iterations each), after a 2^26-double copy that calibrates hardware byte
-counters. --workload lopf: PyPSA's scigrid-de. --workload spmv: the SpMV
-schedules on the 2M-column random LP.
+counters. --workload lopf: PyPSA's scigrid-de. --workload spmv: every SpMV
+schedule on the 2M-column random LP and on the sweep's 256 matrices side by
+side. --workload config2: the 2M-variable random LP in explicit float32 and
+float64 passes under time limits (--pass-seconds).
 """
 from __future__ import annotations
 
@@ -416,7 +419,10 @@ def scenarios(args):
     p = batch[0]
     headline(args, batch, gold,
              {"workload": f"demand scenarios of the 200-bus, 24-snapshot dispatch LP "
-                          f"({p.n} vars, {p.m} rows, {p.nnz} nnz each)", "lps": len(batch)},
+                          f"({p.n} vars, {p.m} rows, {p.nnz} nnz each)", "lps": len(batch),
+              "note": "config 1's LP as a batch of demand scenarios; the uneven sweep of "
+                      "config 3 needs more than 19 minutes to 1e-8 on one MI355X and runs "
+                      "as --workload sweep"},
              sample_by_size=False)
 
 
@@ -536,7 +542,8 @@ def lopf_scigrid(args):
 
 
 SCHEDULES = {0: "auto", 1: "vector-1", 2: "vector-2", 3: "vector-4", 4: "vector-8",
-             5: "vector-16", 6: "vector-32", 7: "vector-64", 8: "row-blocks", 9: "panels"}
+             5: "vector-16", 6: "vector-32", 7: "vector-64", 8: "row-blocks", 9: "panels",
+             10: "row-blocks-small"}
 
 
 def config2_passes(args):

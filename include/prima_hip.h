@@ -109,6 +109,9 @@ int prima_mat_free(prima_ctx* ctx, prima_mat* m);
  * stores the entries cut into (row range, column panel) tiles so that the
  * gathers of x stay in L2. Only available on such matrices. */
 #define PRIMA_SPMV_PANELS 9
+/* The row-block kernel over small blocks (2,048 nonzeros, 1,024 rows), so
+ * that several workgroups share a CU: for matrices of short rows. */
+#define PRIMA_SPMV_ROW_BLOCKS_SMALL 10
 /* Force a schedule for a matrix (PRIMA_EINVAL for an unknown schedule or a
  * panel schedule on a matrix without panels). */
 int prima_mat_set_schedule(prima_ctx* ctx, prima_mat* a, int32_t schedule);

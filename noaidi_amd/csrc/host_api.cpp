@@ -54,6 +54,11 @@ hipError_t op_spmv_panels(int prec, int P, int nranges, const int32_t* range_row
                           void* out, hipStream_t s);
 int panel_rows();
 int panel_tile();
+int small_block_tile();
+int small_block_rows();
+hipError_t op_spmv_blocks_small(int prec, int nblocks, const int32_t* ptr, const int32_t* col,
+                                const void* val, const int32_t* rbd, const void* x, void* out,
+                                hipStream_t s);
 // refine_kernels.hip
 int ref_partial_blocks();
 int ref_result_doubles();
@@ -141,6 +146,8 @@ struct prima_mat {
   void* val = nullptr;
   int32_t* rbd = nullptr;  // row-block descriptors for the row-block schedule
   int32_t nblocks = 0;
+  int32_t* rbd_small = nullptr;  // the same for the small-block schedule
+  int32_t nblocks_small = 0;
   int32_t schedule = 0;    // PRIMA_SPMV_* (0: by mean row length)
   // column-panel layout (PRIMA_SPMV_PANELS); panels == 0 when not built
   int32_t panels = 0, nranges = 0, nlong = 0;
@@ -1845,6 +1852,9 @@ hipError_t spmv_scheduled(prima_ctx* ctx, const prima_mat* a, const void* x, voi
   if (sch == PRIMA_SPMV_ROW_BLOCKS)
     return op_spmv_blocks(ctx->precision, a->nblocks, 1, a->ptr, a->col, a->val, a->rbd, x, out,
                           ctx->stream);
+  if (sch == PRIMA_SPMV_ROW_BLOCKS_SMALL)
+    return op_spmv_blocks_small(ctx->precision, a->nblocks_small, a->ptr, a->col, a->val,
+                                a->rbd_small, x, out, ctx->stream);
   if (sch == PRIMA_SPMV_PANELS)
     return op_spmv_panels(ctx->precision, a->panels, a->nranges, a->range_row0, a->tile_off,
                           a->starts, a->pcol, a->pval, a->nlong, a->long_rows, a->ptr, a->col,
@@ -1958,8 +1968,18 @@ int refine_batch(prima_ctx* ctx, int32_t count, const prima_lp* lps, const prima
     R.obj_norm = norm2(lps[k].c, lps[k].n);
     R.step = red[k].final_step_size;
     R.weight = red[k].final_primal_weight;
-    for (size_t j = 0; j < mats.size() && R.mat < 0; ++j)
-      if (mkey[mats[j].lp - lps] == mkey[k]) R.mat = (int)j;
+    // one device copy per distinct matrix: same content key and, to rule
+    // out a hash collision, the same arrays
+    for (size_t j = 0; j < mats.size() && R.mat < 0; ++j) {
+      const prima_lp* a = mats[j].lp;
+      const prima_lp* b = &lps[k];
+      if (mkey[a - lps] != mkey[k] || a->n != b->n || a->m != b->m || a->nnz != b->nnz) continue;
+      const bool same =
+          (a->m == 0 || std::memcmp(a->row_ptr, b->row_ptr, ((size_t)a->m + 1) * 4) == 0) &&
+          (a->nnz == 0 || (std::memcmp(a->col, b->col, (size_t)a->nnz * 4) == 0 &&
+                           std::memcmp(a->val, b->val, (size_t)a->nnz * 8) == 0));
+      if (same) R.mat = (int)j;
+    }
     if (R.mat < 0) {
       R.mat = (int)mats.size();
       mats.push_back(RefMat{});
@@ -2886,6 +2906,16 @@ int prima_mat_upload_csr(prima_ctx* ctx, int32_t rows, int32_t cols, int64_t nnz
       prima_mat_free(ctx, M);
       return dev_fail(ctx, e, "row-block descriptors");
     }
+    const std::vector<int32_t> rbs =
+        row_blocks(rows, row_ptr, small_block_tile(), small_block_rows());
+    const std::vector<int32_t> ds = block_descriptors(rbs, row_ptr);
+    M->nblocks_small = (int32_t)rbs.size() - 1;
+    e = hipMalloc((void**)&M->rbd_small, ds.size() * 4);
+    if (e == hipSuccess) e = hipMemcpy(M->rbd_small, ds.data(), ds.size() * 4, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      prima_mat_free(ctx, M);
+      return dev_fail(ctx, e, "row-block descriptors");
+    }
   }
   {
     const int prc = build_panels(ctx, M, row_ptr, col, vals);
@@ -2908,6 +2938,7 @@ int prima_mat_free(prima_ctx* ctx, prima_mat* M) {
   if (M->col) (void)hipFree(M->col);
   if (M->val) (void)hipFree(M->val);
   if (M->rbd) (void)hipFree(M->rbd);
+  if (M->rbd_small) (void)hipFree(M->rbd_small);
   for (void* p : {(void*)M->range_row0, (void*)M->tile_off, (void*)M->starts, (void*)M->pcol,
                   M->pval, (void*)M->long_rows})
     if (p) (void)hipFree(p);
@@ -2928,7 +2959,7 @@ int prima_spmv(prima_ctx* ctx, const prima_mat* a, const prima_vec* x, prima_vec
 int prima_mat_set_schedule(prima_ctx* ctx, prima_mat* a, int32_t schedule) {
   int rc = check_open(ctx);
   if (rc) return rc;
-  if (!a || schedule < 0 || schedule > PRIMA_SPMV_PANELS)
+  if (!a || schedule < 0 || schedule > PRIMA_SPMV_ROW_BLOCKS_SMALL)
     return fail(ctx, PRIMA_EINVAL, "unknown SpMV schedule");
   if (schedule == PRIMA_SPMV_PANELS && a->panels == 0)
     return fail(ctx, PRIMA_EINVAL, "matrix has no column-panel layout (x fits in L2)");
@@ -2959,8 +2990,8 @@ int prima_mat_autotune(prima_ctx* ctx, prima_mat* a, int32_t reps, int32_t* chos
   HIP_TRY(ctx, hipMemsetAsync(x, 0, std::max<size_t>(1, (size_t)a->cols * s), ctx->stream));
   double best = INFINITY;
   int pick = PRIMA_SPMV_ROW_BLOCKS;
-  const int last = a->panels > 0 ? PRIMA_SPMV_PANELS : PRIMA_SPMV_ROW_BLOCKS;
-  for (int sch = PRIMA_SPMV_VECTOR_1; sch <= last; ++sch) {
+  for (int sch = PRIMA_SPMV_VECTOR_1; sch <= PRIMA_SPMV_ROW_BLOCKS_SMALL; ++sch) {
+    if (sch == PRIMA_SPMV_PANELS && a->panels == 0) continue;
     HIP_TRY(ctx, spmv_scheduled(ctx, a, x, y, sch));  // warm
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     for (int r = 0; r < reps; ++r) HIP_TRY(ctx, spmv_scheduled(ctx, a, x, y, sch));

@@ -197,6 +197,46 @@ __global__ __launch_bounds__(NT) void k_spmv_blocks(const int32_t* __restrict__ 
               reinterpret_cast<T*>(rsum), rptr, epi);
 }
 
+// The same schedule over small row blocks (SMALL_TILE nonzeros, SMALL_ROWS
+// rows): 28 KB of LDS in float64 instead of 112 KB, so three workgroups fit a
+// CU (the kernel's registers allow three) where the solver's block size fits
+// one; for matrices of short rows with local gathers.
+constexpr int SMALL_TILE = 2048;
+constexpr int SMALL_ROWS = 1024;
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_spmv_blocks_small(const int32_t* __restrict__ ptr,
+                                                          const int32_t* __restrict__ col,
+                                                          const T* __restrict__ val,
+                                                          const T* __restrict__ x,
+                                                          const int32_t* __restrict__ rbd,
+                                                          int nblocks, T* __restrict__ out) {
+  __shared__ T tile[SMALL_TILE];
+  __shared__ T rsum[SMALL_ROWS];
+  __shared__ int rptr[SMALL_ROWS + 1];
+  const int b0 = blockIdx.x;
+  const int b1 = b0 + 1 < nblocks ? b0 + 1 : nblocks;
+  EpiStore<T> epi{out};
+  spmv_blocks<T, EpiStore<T>, SMALL_TILE>(ptr, col, val, x, rbd, b0, b1, tile, rsum, rptr, epi);
+}
+
+int small_block_tile() { return SMALL_TILE; }
+int small_block_rows() { return SMALL_ROWS; }
+
+hipError_t op_spmv_blocks_small(int prec, int nblocks, const int32_t* ptr, const int32_t* col,
+                                const void* val, const int32_t* rbd, const void* x, void* out,
+                                hipStream_t s) {
+  if (nblocks == 0) return hipSuccess;
+  if (prec == 32)
+    k_spmv_blocks_small<float><<<nblocks, NT, 0, s>>>(ptr, col, (const float*)val,
+                                                      (const float*)x, rbd, nblocks, (float*)out);
+  else
+    k_spmv_blocks_small<double><<<nblocks, NT, 0, s>>>(ptr, col, (const double*)val,
+                                                       (const double*)x, rbd, nblocks,
+                                                       (double*)out);
+  return hipGetLastError();
+}
+
 hipError_t op_spmv_blocks(int prec, int nblocks, int bpw, const int32_t* ptr, const int32_t* col,
                           const void* val, const int32_t* rbd, const void* x, void* out,
                           hipStream_t s) {

@@ -36,8 +36,9 @@ constexpr int EPI_ROWS = PRIMA_EPI_ROWS;  // rows whose epilogue operands are in
 // runs the epilogue with the operand loads of four rows in flight per thread.
 // A block holding one row longer than the tile is summed by the whole
 // workgroup. Epilogues are two-phase: load(r) fetches a row's operands,
-// apply(r, sum, ops) consumes them.
-template <typename T, typename Epi>
+// apply(r, sum, ops) consumes them. TL is the capacity of `tile` in entries
+// (the row blocks must have been cut with it).
+template <typename T, typename Epi, int TL = TILE>
 __device__ inline void spmv_blocks(const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
                                    const T* __restrict__ val, const T* __restrict__ xv,
                                    const int32_t* __restrict__ rbd, int b0, int b1, T* tile,
@@ -50,7 +51,7 @@ __device__ inline void spmv_blocks(const int32_t* __restrict__ ptr, const int32_
     const int r0 = d0.x, r1 = d1.x;
     const int p0 = d0.y, p1 = d1.y;
     const int nnz = p1 - p0, nrows = r1 - r0;
-    if (nnz > TILE) {
+    if (nnz > TL) {
       T acc = T(0);
       for (int p = p0 + (int)threadIdx.x; p < p1; p += NT) acc += val[p] * xv[col[p]];
       // fixed-order workgroup tree in the iteration precision

@@ -1,6 +1,7 @@
 """The benchmark's contract on the CPU: it refuses more than one GPU with a
-message, the headline sweep has HiGHS goldens across its horizons, and the
-FP64 denominator samples size strata at their midpoints."""
+message, it offers every workload the measurements name, the uneven sweep
+has HiGHS goldens across its horizons, and the FP64 denominator samples
+size strata at their midpoints."""
 import os
 import subprocess
 import sys
@@ -17,9 +18,18 @@ def test_more_than_one_gpu_is_refused():
     assert r.stdout == ""
 
 
+def test_every_measured_workload_is_offered():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--help"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0
+    for w in ("scenarios", "sweep", "uc-frontier", "kernel", "lopf", "spmv", "config2"):
+        assert w in r.stdout, w
+    assert "--pass-seconds" in r.stdout and "--unbatched" in r.stdout
+
+
 def test_sweep_goldens_cover_every_horizon():
-    # the headline is the uneven sweep: HiGHS goldens include instances of
-    # every snapshot count, and the relative gap is |a-b| / max(1, |b|)
+    # the uneven sweep (--workload sweep): HiGHS goldens include instances
+    # of every snapshot count, and the relative gap is |a-b| / max(1, |b|)
     import bench
     from noaidi_amd import prima
     gold = bench.sweep_goldens()

@@ -153,7 +153,7 @@ def test_panel_schedule_matches_host(hip32, hip64, prec):
     scale = np.abs(A) @ np.abs(x) + 1.0
     t = 1e-12 if prec == 64 else 1e-6
     results = {}
-    for sch in range(1, 10):
+    for sch in range(1, 11):
         k.set_schedule(m, sch)
         k.spmv(m, xv, out)
         got = k.download(out)
@@ -164,7 +164,41 @@ def test_panel_schedule_matches_host(hip32, hip64, prec):
     k.spmv(m, xv, out)
     assert np.array_equal(k.download(out), results[9])
     choice, ms = k.autotune(m, 3)
-    assert 1 <= choice <= 9 and ms > 0
+    assert 1 <= choice <= 10 and ms > 0
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_row_block_schedules_cover_long_and_short_rows(hip32, hip64, prec):
+    # short dispatch-like rows, a run of empty rows, rows exactly at the small
+    # schedule's block capacity (2,048) and rows beyond it (summed by the
+    # whole workgroup), more rows than one small block holds: the row-block
+    # schedules (solver-sized and small) agree with the f64 host product and
+    # give the same bits twice
+    import scipy.sparse as sp
+    k = hip32 if prec == 32 else hip64
+    rng = np.random.default_rng(prec)
+    rows, cols = 7000, 50_000
+    lens = rng.integers(3, 7, rows)
+    lens[100:140] = 0
+    lens[[7, 2000]] = 2048
+    lens[[11, 3001, 6999]] = [2049, 5000, 9000]
+    ptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    col = rng.integers(0, cols, int(ptr[-1])).astype(np.int32)
+    val = rng.standard_normal(int(ptr[-1]))
+    A = sp.csr_matrix((val, col, ptr), shape=(rows, cols))
+    x = rng.standard_normal(cols)
+    m = k.uploadMatrix(rows, cols, ptr, col, val)
+    xv, out = k.upload(x), k.allocate(rows)
+    ref = A @ x
+    scale = np.abs(A) @ np.abs(x) + 1.0
+    t = 1e-12 if prec == 64 else 1e-6
+    for sch in (8, 10):
+        k.set_schedule(m, sch)
+        k.spmv(m, xv, out)
+        got = k.download(out)
+        assert np.all(np.abs(got - ref) <= 10 * t * scale), sch
+        k.spmv(m, xv, out)
+        assert np.array_equal(k.download(out), got), sch
 
 
 def test_panel_schedule_needs_a_wide_matrix(hip64):
@@ -172,4 +206,4 @@ def test_panel_schedule_needs_a_wide_matrix(hip64):
     with pytest.raises(ValueError):
         hip64.set_schedule(m, 9)
     with pytest.raises(ValueError):
-        hip64.set_schedule(m, 10)
+        hip64.set_schedule(m, 11)

@@ -213,6 +213,15 @@ def test_iterative_refinement_path(hip32):
     for name, status in (("infeasible", "PrimalInfeasible"), ("unbounded", "DualInfeasible")):
         p, *_ = fixture(name)
         assert hip32.mixed_solve(p, P9, ir).status == status
+    # strongly active bounds fixed in the correction LPs: same answers
+    fixed = prima.MixedParams(refinement="iterative", fix_threshold=1.0)
+    for name in OPTIMAL:
+        p, eobj, _, _ = fixture(name)
+        s = hip32.mixed_solve(p, P9, fixed)
+        assert s.status == "Optimal", (name, s.status, s.mixed)
+        assert abs(s.objective_value - eobj) <= 1e-6 * max(1, abs(eobj)), name
+    with pytest.raises(ValueError):
+        hip32.mixed_solve(p, P9, prima.MixedParams(refinement="iterative", fix_threshold=-1.0))
 
 
 def test_mixed_precision_fp64_device_skips_reduced_pass(hip64):
