@@ -135,7 +135,8 @@ def expand_standard_types(net: Network) -> Network:
             t = lib[name]
             par = trafos.float("num_parallel", e)
             if not par > 0.0:
-                raise UnknownType(f"Transformer '{e}' has num_parallel = {par}")
+                raise UnknownType(f"Transformer '{e}': num_parallel {par} must be positive to "
+                                  "scale its standard type's impedance")
             s_nom = t.float("s_nom", name)
             if not s_nom > 0.0:
                 raise UnknownType(f"transformer type '{name}' has s_nom = {s_nom}")
@@ -323,8 +324,8 @@ def _reject_expansion(net: Network):
             continue
         for e in extendables(t):
             if "overnight_cost" in t.static and math.isfinite(t.float("overnight_cost", e)):
-                raise UnsupportedNetwork(f"{component} '{e}' sets overnight_cost, which PyPSA "
-                                         "annuitises; only capital_cost is modelled")
+                raise UnsupportedNetwork(f"{component} '{e}': overnight_cost is given, but PyPSA would "
+                                         "turn it into an annuity first; give capital_cost instead")
             mod = f"{attr}_mod"
             if mod in t.static and t.float(mod, e) != 0.0:
                 raise UnsupportedNetwork(f"{component} '{e}': {mod} makes the capacity a whole "
@@ -388,13 +389,13 @@ def _reject_storage(table: Table):
     for a in ("p_set", "p_dispatch_set", "p_store_set"):
         p = _pinned(table, a)
         if p:
-            raise UnsupportedNetwork(f"StorageUnit '{p[0]}' sets {a}, which pins the dispatch at a "
-                                     "snapshot; that constraint is not built here")
+            raise UnsupportedNetwork(f"StorageUnit '{p[0]}': {a} fixes its power in a snapshot, "
+                                     "and no such equality is generated")
     for a in ("cyclic_state_of_charge_per_period", "state_of_charge_initial_per_period"):
         f = _flagged(table, a)
         if f:
-            raise UnsupportedNetwork(f"StorageUnit '{f[0]}' sets {a}, a multi-investment-period "
-                                     "concept; this model has a single horizon")
+            raise UnsupportedNetwork(f"StorageUnit '{f[0]}': {a} belongs to PyPSA's investment "
+                                     "periods, and the LP here spans one period only")
     q = _quadratic(table)
     if q:
         raise UnsupportedNetwork(f"StorageUnit '{q[0]}': marginal_cost_quadratic is non-zero, "
@@ -463,15 +464,16 @@ def build(network: Network) -> Model:
     dangling = dangling_bus_references(net)
     if dangling:
         c, e, port, bus = dangling[0]
-        raise UnsupportedNetwork(f"{c} '{e}' references unknown bus '{bus}' via {port}")
+        raise UnsupportedNetwork(f"{c} '{e}': its {port} names bus '{bus}', which the network "
+                                 "does not define")
     _reject_expansion(net)
     T = len(net.snapshots)
     if T == 0:
-        raise UnsupportedNetwork("network has no snapshots")
+        raise UnsupportedNetwork("the network defines no snapshot to dispatch")
     bus_table = net.table("Bus")
     buses = bus_table.ids if bus_table else []
     if not buses:
-        raise UnsupportedNetwork("network has no buses")
+        raise UnsupportedNetwork("the network defines no bus")
 
     def tables_with(r):
         return [t for t in net.tables.values() if role(t.spec) == r and t.size > 0]
@@ -623,8 +625,9 @@ def build(network: Network) -> Model:
                 eh = net.weighting("stores", s)
                 eff_d = storage.value_at("efficiency_dispatch", e, s)
                 if not eff_d > 0.0:
-                    raise UnsupportedNetwork(f"StorageUnit '{e}' has efficiency_dispatch = {eff_d} "
-                                             f"at snapshot {s}, which the energy balance divides by")
+                    raise UnsupportedNetwork(f"StorageUnit '{e}': efficiency_dispatch {eff_d} in snapshot "
+                                             f"{s} is not positive, and the state-of-charge row "
+                                             "divides by it")
                 stand = math.pow(1.0 - storage.value_at("standing_loss", e, s), eh)
                 terms = [(columns[(SOC, e, s)], 1.0),
                          (columns[(DISPATCH, e, s)], eh / eff_d),
@@ -647,9 +650,9 @@ def build(network: Network) -> Model:
     for carrier, terms in cycles:
         if not any(imp[(comp, e)] != 0.0 for comp, e, _ in terms):
             raise UnsupportedNetwork(
-                "every branch in the cycle " + ", ".join(f"{c} '{e}'" for c, e, _ in terms)
-                + f" has zero impedance in its {carrier} sub-network, so Kirchhoff's voltage law "
-                "would impose nothing")
+                f"a cycle of the {carrier} sub-network has zero impedance on all of its branches ("
+                + ", ".join(f"{c} '{e}'" for c, e, _ in terms)
+                + "): its voltage-law row would be empty")
     for s in range(T):
         for carrier, terms in cycles:
             b.equalityConstraint([(columns[(comp, e, s)], o * imp[(comp, e)])

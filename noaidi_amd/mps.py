@@ -58,7 +58,7 @@ _JAVA_DOUBLE = re.compile(
 def _number(token: str, line: str) -> float:
     t = token.strip()
     if not _JAVA_DOUBLE.fullmatch(t):
-        raise MpsParseException(f"'{token}' is not a number, in: '{line}'")
+        raise MpsParseException(f"cannot read '{token}' as a number (line '{line}')")
     if t[-1] in "fFdD" and not t.endswith("Infinity"):
         t = t[:-1]
     if t.lstrip("+-") == "NaN":
@@ -130,7 +130,7 @@ def from_lines(lines: Iterable[str], fallback_name: str = "inline") -> MpsInstan
             return None
         r = rows.get(rname)
         if r is None:
-            raise MpsParseException(f"{context} refers to unknown row '{rname}'")
+            raise MpsParseException(f"{context} names row '{rname}', which ROWS does not declare")
         return r
 
     for raw in lines:
@@ -147,7 +147,7 @@ def from_lines(lines: Iterable[str], fallback_name: str = "inline") -> MpsInstan
             continue
         if section == "ROWS":
             if len(tokens) < 2:
-                raise MpsParseException(f"malformed ROWS entry: '{line}'")
+                raise MpsParseException(f"ROWS line has the wrong number of fields: '{line}'")
             kind, rname = tokens[0].upper(), tokens[1]
             if kind == "N":
                 if not objective:
@@ -167,7 +167,7 @@ def from_lines(lines: Iterable[str], fallback_name: str = "inline") -> MpsInstan
                     integer_block = False
                 continue
             if len(tokens) < 3:
-                raise MpsParseException(f"malformed COLUMNS entry: '{line}'")
+                raise MpsParseException(f"COLUMNS line has the wrong number of fields: '{line}'")
             col = column(tokens[0])
             if integer_block:
                 col.integer = True
@@ -193,7 +193,7 @@ def from_lines(lines: Iterable[str], fallback_name: str = "inline") -> MpsInstan
                     r.range = value
         elif section == "BOUNDS":
             if len(tokens) < 2:
-                raise MpsParseException(f"malformed BOUNDS entry: '{line}'")
+                raise MpsParseException(f"BOUNDS line has the wrong number of fields: '{line}'")
             kind = tokens[0].upper()
             # the column is whichever candidate token names a declared column:
             # the set name is optional and valueless kinds may carry a value
@@ -232,7 +232,7 @@ def from_lines(lines: Iterable[str], fallback_name: str = "inline") -> MpsInstan
             elif kind == "UI":
                 col.upper, col.integer = value(), True
             else:
-                raise MpsParseException(f"unknown bound type '{kind}' on '{cname}'")
+                raise MpsParseException(f"bound type '{kind}' (column '{cname}') is not one of the MPS types")
         elif section == "OBJSENSE":
             _check_objsense(tokens[0])
         elif section in ("ENDATA", ""):
@@ -264,7 +264,7 @@ def _build(name, obj_constant, rows, cols) -> MpsInstance:
     for c in cols:
         b.objectiveCoefficient(c.index, c.objective)
         if c.lower > c.upper:
-            raise MpsParseException(f"column '{c.name}' has empty bounds [{c.lower}, {c.upper}]")
+            raise MpsParseException(f"column '{c.name}': lower bound {c.lower} exceeds upper bound {c.upper}")
         b.bounds(c.index, c.lower, c.upper)
     for r in rows:
         lo, hi = _bracket(r)

@@ -76,7 +76,7 @@ def _kind(pypsa_type: str) -> str:
                          ("series", "float")):
         if t.startswith(prefix):
             return kind
-    raise ValueError(f"PyPSA attribute type '{pypsa_type}' is not one this reader knows")
+    raise ValueError(f"this reader cannot parse attributes of type '{pypsa_type}'")
 
 
 def _default(raw, kind: str):
@@ -544,7 +544,7 @@ def sub_networks(net: Network):
             ends = [b for b in ends if b]
             for b in ends:
                 if b not in parent:
-                    raise MalformedNetwork(f"{table.name} '{e}' references unknown bus '{b}'")
+                    raise MalformedNetwork(f"{table.name} '{e}' names bus '{b}', which the network does not define")
             for i in range(len(ends)):
                 for j in range(i + 1, len(ends)):
                     ra, rb = find(ends[i]), find(ends[j])

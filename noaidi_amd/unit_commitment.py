@@ -14,8 +14,8 @@ whenever u is integral, and priced non-negatively, so they need no branching):
 
 Bus balance holds per bus with no flows, so a network with any branch is
 refused, as are global constraints, storage, ramp limits, stand-by and
-quadratic costs and extendable generators (each would otherwise be dropped and
-price the schedule below the truth).
+quadratic costs and extendable generators (leaving any of them out would make
+the schedule look cheaper than it is).
 """
 from __future__ import annotations
 
@@ -48,8 +48,9 @@ def _reject(net: Network):
     gc = net.table("GlobalConstraint")
     if gc is not None and gc.size > 0:
         raise UnsupportedNetwork(
-            f"network has {gc.size} global constraint(s); emissions and primary-energy caps are "
-            "not modelled here, and dropping one would price the schedule below the truth")
+            f"network has {gc.size} global constraint(s): emission and primary-energy caps have "
+            "no rows in this model, and leaving one out would make the schedule look cheaper "
+            "than it is")
     unhandled = [t for t in net.tables.values() if t.size > 0 and t.name not in HANDLED]
     if unhandled:
         raise UnsupportedNetwork(
@@ -58,7 +59,8 @@ def _reject(net: Network):
     dangling = dangling_bus_references(net)
     if dangling:
         c, e, port, bus = dangling[0]
-        raise UnsupportedNetwork(f"{c} '{e}' references unknown bus '{bus}' via {port}")
+        raise UnsupportedNetwork(f"{c} '{e}': its {port} names bus '{bus}', which the network "
+                                 "does not define")
     g = net.table("Generator")
     if g is None:
         return
@@ -103,7 +105,7 @@ def build(network: Network) -> UcModel:
     _reject(net)
     T = len(net.snapshots)
     if T == 0:
-        raise UnsupportedNetwork("network has no snapshots")
+        raise UnsupportedNetwork("the network defines no snapshot to schedule")
     g = net.require("Generator")
     buses = net.require("Bus").ids
     units = []
