@@ -46,3 +46,22 @@ def test_stratified_sample_takes_strata_midpoints():
     # sorted sizes 10 20 30 40 | 50 60 70 80: strata of two, midpoints the upper
     assert [sizes[i] for i in pick] == [20, 40, 60, 80]
     assert len(set(bench.stratified(list(range(256)), 4))) == 4
+
+
+def test_roofline_arithmetic():
+    # the dominant instantiation by device time; achieved = algorithmic bytes
+    # per launch / average launch duration; frac against 8 TB/s; traffic is
+    # the PMC factor times the bytes per launch (float64 only has one)
+    import bench
+    stats = {"kernel_ms_by_precision": [10.0, 40.0], "launches_by_precision": [2, 4],
+             "algorithmic_bytes_by_precision": [1e9, 8e10], "kernel_ms": 50.0, "launches": 6,
+             "algorithmic_bytes": 8.1e10}
+    r = bench.roofline(stats)
+    assert r["kernel"] == "pdhg_persistent<double>" and r["launches"] == 4
+    assert abs(r["avg_launch_ms"] - 10.0) < 1e-12
+    assert abs(r["algorithmic_bytes_per_launch"] - 2e10) < 1e-3
+    assert abs(r["achieved"] - 2000.0) < 1e-9 and abs(r["frac"] - 0.25) < 1e-12
+    assert abs(r["traffic"] - bench.TRAFFIC_PER_ALGORITHMIC_BYTE[64] * 2e10) < 1e-3
+    f32 = dict(stats, kernel_ms_by_precision=[40.0, 10.0])
+    r = bench.roofline(f32)
+    assert r["kernel"] == "pdhg_persistent<float>" and r["traffic"] is None
