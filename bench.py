@@ -37,7 +37,9 @@ iterations each), after a 2^26-double copy that calibrates hardware byte
 counters. --workload lopf: PyPSA's scigrid-de. --workload spmv: every SpMV
 schedule on the 2M-column random LP and on the sweep's 256 matrices side by
 side. --workload config2: the 2M-variable random LP in explicit float32 and
-float64 passes under time limits (--pass-seconds).
+float64 passes under time limits (--pass-seconds). --workload tune: the
+solver autotuner's timings on config 1's LP, the sweep's largest LP and
+config 2's LP (--iterations per candidate).
 """
 from __future__ import annotations
 
@@ -546,6 +548,32 @@ SCHEDULES = {0: "auto", 1: "vector-1", 2: "vector-2", 3: "vector-4", 4: "vector-
              10: "row-blocks-small"}
 
 
+def tune_report(args):
+    """The solver autotuner (prima_tune_solver) on three matrix families: the
+    200-bus dispatch LP (config 1), the largest LP of the uneven sweep and
+    the 2M-variable random LP; --iterations accepted iterations per candidate
+    in float64. Prints every candidate's device time per iteration."""
+    cases = [("dispatch 200 buses x 24 snapshots (config 1)", lambda: prima.config1()),
+             ("sweep LP 49 (1,988 buses x 168 snapshots)", lambda: prima.config3(49)),
+             ("random 2M x 1M (config 2)", lambda: prima.config2())]
+    out = {"iterations_per_candidate": args.iterations, "precision": 64, "cases": []}
+    with prima.HipKernels(0, 64) as k:
+        for name, make in cases:
+            p = make()
+            t = time.perf_counter()
+            r = k.tune_solver(p, prima.PdhgParams(), iterations=args.iterations)
+            rec = {"lp": name, "n": p.n, "m": p.m, "nnz": p.nnz,
+                   "chosen": {"workgroups": r["workgroups"],
+                              "blocks_per_workgroup": r["blocks_per_workgroup"],
+                              "us_per_iteration": r["us_per_iteration"]},
+                   "candidates": [{"workgroups": g, "blocks_per_workgroup": b,
+                                   "us_per_iteration": u} for g, b, u in r["candidates"]],
+                   "seconds": time.perf_counter() - t}
+            log(json.dumps(rec))
+            out["cases"].append(rec)
+    print(json.dumps(out), flush=True)
+
+
 def config2_passes(args):
     """The large random LP (2M variables, 1M rows, 20M nonzeros, lognormal
     row lengths capped at 2048, known optimum c'x* = b'y*) in explicit
@@ -669,7 +697,7 @@ def main():
                     help="time limits of the three passes of --workload config2")
     ap.add_argument("--workload", default="scenarios",
                     choices=["sweep", "scenarios", "uc-frontier", "kernel", "lopf", "spmv",
-                             "config2"])
+                             "config2", "tune"])
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on exactly one GPU: this build solves independent LPs per GPU "
@@ -677,7 +705,7 @@ def main():
         sys.exit(2)
     {"sweep": sweep, "scenarios": scenarios, "uc-frontier": uc_frontier,
      "kernel": kernel_profile, "lopf": lopf_scigrid, "spmv": spmv_roofline,
-     "config2": config2_passes}[args.workload](args)
+     "config2": config2_passes, "tune": tune_report}[args.workload](args)
 
 
 if __name__ == "__main__":
