@@ -354,7 +354,7 @@ def test_uneven_sweep_horizons_against_highs(hip32):
     # mixed-precision batch; every LP Optimal, within 1e-8 of HiGHS, and its
     # returned point meets the KKT test on the original data
     with open(os.path.join(os.path.dirname(__file__), "golden", "highs_sweep.json")) as f:
-        gold = [d for d in json.load(f)[20:] if d["status"] == 0]
+        gold = [d for d in json.load(f)[20:] if d["status"] == 0 and d["n"] <= 400_000]
     probs = [prima.config3(d["args"]["k"]) for d in gold]
     snaps = {d["config3_args"]["snapshots"] for d in gold}
     assert snaps == {24, 72, 168}, snaps
