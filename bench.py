@@ -3,9 +3,8 @@
 Headline workload (the default, --workload scenarios): 256 demand scenarios
 of the 200-bus, 24-snapshot PyPSA-shaped dispatch LP (the config-1 LP: 34,800
 variables, 14,400 rows, 64,750 nonzeros; per-bus demand x U[0.8, 1.2] with
-seed 1000 + k). The metric is a batched one and config 1 is its single-GPU
-configuration; the uneven sweep (config 3) takes more than 19 minutes to
-1e-8 and does not fit a 600 s run. One step solves all 256 in one batched
+seed 1000 + k). The metric is a batched one; this batch finishes inside a
+600 s run, while the uneven 256-LP sweep takes more than 19 minutes to 1e-8. One step solves all 256 in one batched
 call with mixed precision: a float32 device pass, then a float64 device pass
 warm-started from it (the reference's MixedPrecision contract), every LP in the same
 persistent launches with its own step size, restarts and termination. Every
@@ -17,7 +16,7 @@ objective (tests/golden/highs_dispatch.json), or the run fails.
 in {24, 72, 168}; 88.1M variables over 256 LPs). Its largest LPs (up to 2.4M
 variables) keep the batch running for more than 19 minutes.
 
-Timing follows the driver's fixed command (--steps K --warmup W): warm-up is
+Timing follows a fixed harness command (--steps K --warmup W): warm-up is
 capped at one small solve per context (the persistent kernel has no JIT to
 warm), and steps are timed until the wall budget (--budget seconds) would be
 exceeded, at least one; `steps` reports the steps actually timed, and a
@@ -28,7 +27,8 @@ metric's denominator) on a sample stratified by the FP64 iteration counts
 CPU path) on the host cores. One JSON line, printed and flushed at the end.
 
   python bench.py [--steps K] [--warmup W] [--budget S] [--no-cpu]
-                  [--workload scenarios|sweep|uc-frontier|kernel|lopf|spmv|config2]
+                  [--workload scenarios|sweep|sweep-fp64|uc-frontier|kernel|lopf|spmv|
+                              random-lp|tune]
 
 --workload uc-frontier: the 100-bus UC relaxation's depth-9 branch-and-bound
 frontier (--batch nodes, shared K, warm-started from the root). --workload
@@ -36,10 +36,10 @@ kernel: the solver kernel alone for profiling (dispatch LPs for --iterations
 iterations each), after a 2^26-double copy that calibrates hardware byte
 counters. --workload lopf: PyPSA's scigrid-de. --workload spmv: every SpMV
 schedule on the 2M-column random LP and on the sweep's 256 matrices side by
-side. --workload config2: the 2M-variable random LP in explicit float32 and
+side. --workload random-lp: the 2M-variable random LP in explicit float32 and
 float64 passes under time limits (--pass-seconds). --workload tune: the
-solver autotuner's timings on config 1's LP, the sweep's largest LP and
-config 2's LP (--iterations per candidate).
+solver autotuner's timings on the 200-bus dispatch LP, the sweep's largest
+LP and the 2M-variable random LP (--iterations per candidate).
 """
 from __future__ import annotations
 
@@ -455,9 +455,9 @@ def scenarios(args):
     headline(args, batch, gold,
              {"workload": f"demand scenarios of the 200-bus, 24-snapshot dispatch LP "
                           f"({p.n} vars, {p.m} rows, {p.nnz} nnz each)", "lps": len(batch),
-              "note": "config 1's LP as a batch of demand scenarios; the uneven sweep of "
-                      "config 3 needs more than 19 minutes to 1e-8 on one MI355X and runs "
-                      "as --workload sweep"},
+              "note": "the 200-bus LP as a batch of demand scenarios; the uneven 256-LP "
+                      "sweep needs more than 19 minutes to 1e-8 on one MI355X and runs as "
+                      "--workload sweep"},
              sample_by_size=False)
 
 
@@ -583,12 +583,12 @@ SCHEDULES = {0: "auto", 1: "vector-1", 2: "vector-2", 3: "vector-4", 4: "vector-
 
 def tune_report(args):
     """The solver autotuner (prima_tune_solver) on three matrix families: the
-    200-bus dispatch LP (config 1), the largest LP of the uneven sweep and
+    200-bus dispatch LP, the largest LP of the uneven sweep and
     the 2M-variable random LP; --iterations accepted iterations per candidate
     in float64. Prints every candidate's device time per iteration."""
-    cases = [("dispatch 200 buses x 24 snapshots (config 1)", lambda: prima.config1()),
+    cases = [("dispatch 200 buses x 24 snapshots", lambda: prima.config1()),
              ("sweep LP 49 (1,988 buses x 168 snapshots)", lambda: prima.config3(49)),
-             ("random 2M x 1M (config 2)", lambda: prima.config2())]
+             ("random 2M x 1M", lambda: prima.config2())]
     out = {"iterations_per_candidate": args.iterations, "precision": 64, "cases": []}
     with prima.HipKernels(0, 64) as k:
         for name, make in cases:
@@ -607,7 +607,7 @@ def tune_report(args):
     print(json.dumps(out), flush=True)
 
 
-def config2_passes(args):
+def random_lp_passes(args):
     """The large random LP (2M variables, 1M rows, 20M nonzeros, lognormal
     row lengths capped at 2048, known optimum c'x* = b'y*) in explicit
     mixed-precision passes, each under a time limit: a float32 pass to 1e-6,
@@ -616,7 +616,7 @@ def config2_passes(args):
     and the relative error of its objective against the known optimum."""
     t = time.perf_counter()
     p = prima.config2()
-    log(f"config 2: n={p.n} m={p.m} nnz={p.nnz} known={p.known_objective!r} "
+    log(f"random LP: n={p.n} m={p.m} nnz={p.nnz} known={p.known_objective!r} "
         f"(built in {time.perf_counter() - t:.1f} s)")
     lim = args.pass_seconds
 
@@ -628,7 +628,7 @@ def config2_passes(args):
                 "primal_residual": s.primal_residual, "dual_residual": s.dual_residual,
                 "workgroups": s.workgroups}
 
-    out = {"workload": f"config 2: random LP {p.n} vars, {p.m} rows, {p.nnz} nnz",
+    out = {"workload": f"random LP: {p.n} vars, {p.m} rows, {p.nnz} nnz",
            "known_objective": p.known_objective, "passes": []}
     with prima.HipKernels(0, 32) as k32, prima.HipKernels(0, 64) as k64:
         plan = [(k32, "float32 to 1e-6", 1e-6, lim[0]), (k64, "float64 to 1e-6, warm", 1e-6, lim[1]),
@@ -727,10 +727,10 @@ def main():
     ap.add_argument("--iterations", type=int, default=2048,
                     help="iterations per LP for --workload kernel")
     ap.add_argument("--pass-seconds", type=float, nargs=3, default=[240.0, 186.7, 373.3],
-                    help="time limits of the three passes of --workload config2")
+                    help="time limits of the three passes of --workload random-lp")
     ap.add_argument("--workload", default="scenarios",
                     choices=["sweep", "sweep-fp64", "scenarios", "uc-frontier", "kernel", "lopf",
-                             "spmv", "config2", "tune"])
+                             "spmv", "random-lp", "tune"])
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on exactly one GPU: this build solves independent LPs per GPU "
@@ -738,7 +738,7 @@ def main():
         sys.exit(2)
     {"sweep": sweep, "sweep-fp64": sweep_fp64, "scenarios": scenarios, "uc-frontier": uc_frontier,
      "kernel": kernel_profile, "lopf": lopf_scigrid, "spmv": spmv_roofline,
-     "config2": config2_passes, "tune": tune_report}[args.workload](args)
+     "random-lp": random_lp_passes, "tune": tune_report}[args.workload](args)
 
 
 if __name__ == "__main__":

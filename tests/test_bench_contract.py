@@ -22,7 +22,7 @@ def test_every_measured_workload_is_offered():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--help"],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0
-    for w in ("scenarios", "sweep", "sweep-fp64", "uc-frontier", "kernel", "lopf", "spmv", "config2",
+    for w in ("scenarios", "sweep", "sweep-fp64", "uc-frontier", "kernel", "lopf", "spmv", "random-lp",
               "tune"):
         assert w in r.stdout, w
     assert "--pass-seconds" in r.stdout and "--unbatched" in r.stdout
