@@ -1551,9 +1551,6 @@ int run_jobs(prima_ctx* ctx, int prec, std::vector<SolveJob>& jobs, const prima_
   return run_wave(ctx, prec, jobs, wave, p, certificates, capacity);
 }
 
-int prepare_job(prima_ctx* ctx, SolveJob& J, const prima_lp* lp, const prima_params* p,
-                const prima_warm* warm);
-
 // Runs body(i) for i in [0, count) on up to 16 host threads (the host
 // preparation of a batch is independent per LP).
 template <typename F>
@@ -1671,18 +1668,6 @@ int prepare_jobs(prima_ctx* ctx, std::vector<SolveJob>& jobs,
     h.obj_norm = norm2(lp->c, lp->n);
     J.prep_ms = ms_since(t0[j]);
   });
-  return PRIMA_OK;
-}
-
-int prepare_job(prima_ctx* ctx, SolveJob& J, const prima_lp* lp, const prima_params* p,
-                const prima_warm* warm) {
-  std::vector<SolveJob> one(1);
-  const int rc = prepare_jobs(ctx, one, {lp}, p, {warm});
-  if (rc) return rc;
-  J.lp = one[0].lp;
-  J.warm = one[0].warm;
-  J.h = std::move(one[0].h);
-  J.prep_ms = one[0].prep_ms;
   return PRIMA_OK;
 }
 
