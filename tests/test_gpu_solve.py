@@ -370,6 +370,27 @@ def test_uneven_sweep_horizons_against_highs(hip32):
         assert_kkt(p, s)
 
 
+@pytest.mark.timeout(300)
+def test_random_2m_lp_evaluation_matches_oracle(hip64):
+    # the 2M-variable random LP (1M rows, 20M nonzeros, lognormal row lengths
+    # capped at 2048, uniform random columns) on the whole device for a fixed
+    # number of iterations: the iteration limit is reported, and the residual
+    # and objectives the device evaluated are those the oracle's Kkt.evaluate
+    # recomputes from the returned vectors alone, which lie in their box
+    p = prima.config2()
+    s = hip64.solve(p, prima.PdhgParams(max_iterations=2_000))
+    assert s.status == "IterationLimit" and s.iterations == 2_000
+    assert s.workgroups >= 128
+    e = oracle_of(p).kkt(s.primal, s.dual)
+    assert np.all(np.isfinite(e)), e
+    assert abs(e[2] - s.objective_value) <= 1e-9 * max(1.0, abs(e[2]))
+    assert abs(e[3] - s.dual_objective_value) <= 1e-6 * max(1.0, abs(e[3]))
+    assert abs(e[0] - s.primal_residual) <= 1e-6 * max(1.0, e[0])
+    assert abs(e[1] - s.dual_residual) <= 1e-6 * max(1.0, e[1])
+    room = 1e-12 * (1.0 + np.abs(s.primal))
+    assert np.all(s.primal >= p.l - room) and np.all(s.primal <= p.u + room)
+
+
 def test_solver_autotuner():
     # prima_tune_solver on the 200-bus dispatch LP (65k nonzeros, so group
     # sizes 32, 64 and 128, each with three row-block cuts): every candidate
